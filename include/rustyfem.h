@@ -1,0 +1,116 @@
+/*
+ * rustyfem-mi355x: the C solver ABI of RustyFEM's ffi/ library, served by
+ * hand-written HIP kernels on one MI355X.
+ *
+ * The eight rustyfem_* entry points below keep the reference's signatures and
+ * semantics exactly (ffi/src/lib.rs); a caller built against the reference's
+ * librustyfem_ffi binds them unchanged.  The functions marked "additive" are
+ * new symbols; no reference signature changed.
+ *
+ * Input of every factorization: a symmetric (or, for _unsym, unsymmetric)
+ * matrix in CSC form in the caller's numbering (both triangles, or for the
+ * symmetric modes also only the lower one), the node of every DOF (loc, n
+ * entries) and the node coordinates (coords, 3*nn entries).  Ordering is
+ * geometric nested dissection at node level with 16 nodes per leaf.
+ */
+#ifndef RUSTYFEM_H
+#define RUSTYFEM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct Handle Handle; /* opaque; owned by the library until rustyfem_free */
+
+/* Positive definite multifrontal LDL^T.  Returns NULL on error: *err_dof is the
+ * caller DOF of the first non-positive pivot, or -1 for invalid input.  d_out
+ * (n entries, may be NULL) receives the pivots D in caller order.
+ * Replaces ffi/src/lib.rs:64-97 (rustyfem_factor). */
+Handle *rustyfem_factor(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                        const int64_t *indices, const double *data, double *d_out, int64_t *err_dof);
+
+/* Symmetric indefinite LDL^T with 1x1/2x2 pivots and delayed pivots.
+ * inertia_out (3 entries, may be NULL) = [negative, zero, positive].
+ * Replaces ffi/src/lib.rs:272-290 (rustyfem_factor_indef). */
+Handle *rustyfem_factor_indef(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                              const int64_t *indices, const double *data, int64_t *inertia_out, int64_t *err_dof);
+
+/* Unsymmetric multifrontal LU on the symmetrized pattern (both triangles given).
+ * Replaces ffi/src/lib.rs:299-311 (rustyfem_factor_unsym). */
+Handle *rustyfem_factor_unsym(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                              const int64_t *indices, const double *data, int64_t *err_dof);
+
+/* Inertia [negative, zero, positive]; 0, or -1 for an LU handle.
+ * Replaces ffi/src/lib.rs:319-329 (rustyfem_inertia). */
+int32_t rustyfem_inertia(const Handle *h, int64_t *out);
+
+/* Solves K X = B in place for nrhs right-hand sides stored column by column
+ * (n each).  0 on success, -1 on an internal error.
+ * Replaces ffi/src/lib.rs:336-371 (rustyfem_solve). */
+int32_t rustyfem_solve(const Handle *h, size_t nrhs, double *b);
+
+/* out[0] = estimated flops of the factorization, out[1] = stored entries of L.
+ * Replaces ffi/src/lib.rs:377-383 (rustyfem_info). */
+void rustyfem_info(const Handle *h, double *out);
+
+/* Internal equation of every caller DOF; uncoupled DOFs numbered last.
+ * Replaces ffi/src/lib.rs:390-398 (rustyfem_order). */
+void rustyfem_order(const Handle *h, int64_t *out);
+
+/* Frees a handle (NULL-safe, at most once).  Replaces ffi/src/lib.rs:402-407. */
+void rustyfem_free(Handle *h);
+
+/* ---------------------------------------------------------------- additive */
+
+/* Options of the additive entry points. */
+typedef struct rustyfem_options {
+  int32_t precision;  /* 0: FP64 factor (the reference's arithmetic class);
+                         1: mixed -- FP32 Schur updates on MFMA with FP32
+                            accumulation, FP64 pivot blocks, FP64 refinement */
+  int32_t leaf;       /* nodes per nested-dissection leaf; 0 = 16 (ffi/src/lib.rs:17) */
+  int32_t max_refine; /* refinement corrections per solve; -1 = default
+                         (0 for precision 0, 30 for precision 1) */
+  int32_t reserved;
+  double refine_tol;  /* refinement target ||b - A x|| / ||b|| (0 = 1e-10) */
+} rustyfem_options;
+
+/* rustyfem_factor with options (additive; no reference counterpart). */
+Handle *rustyfem_factor_ex(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                           const int64_t *indices, const double *data, const rustyfem_options *opt,
+                           double *d_out, int64_t *err_dof);
+
+/* New values on the same pattern (the caller's indptr/indices of the first
+ * factorization), refactored on the GPU.  data == NULL refactors the values
+ * already resident in device memory.  0 on success, 1 on a failing pivot
+ * (*err_dof set), -1 on an internal error.  Mirrors the reference's
+ * SystemMatrix zero/add/factor cycle (src/solver/api.rs:12-60) (additive). */
+int32_t rustyfem_refactor(Handle *h, const double *data, int64_t *err_dof);
+
+/* Solve with device pointers (caller numbering, column-major n x nrhs, in
+ * place); runs refinement when the handle's options ask for it (additive). */
+int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
+
+/* Statistics of the last factorization / solve (additive):
+ * out[0] factor ms, [1] solve ms, [2] refined-solve ms, [3] refinement steps,
+ * [4] relative residual after refinement (-1 if not refined), [5] flops,
+ * [6] device bytes held, [7] precision, [8] number of fronts, [9] levels. */
+void rustyfem_stats(const Handle *h, double *out, size_t len);
+
+/* Last error message of the calling thread ("" if none) (additive). */
+const char *rustyfem_last_error(void);
+
+/* Host-only symbolic analysis of the positive definite path (additive): the
+ * ordering rustyfem_order would report (n entries) and rustyfem_info's
+ * [flops, factor entries], without touching the GPU.  0 on success, 1 when
+ * the input is rejected before the factorization (*err_dof as for
+ * rustyfem_factor), -1 on an internal error.  leaf <= 0 means 16. */
+int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                         const int64_t *indices, const double *data, int32_t leaf, int64_t *order_out,
+                         double *info_out, int64_t *err_dof);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

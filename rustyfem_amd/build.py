@@ -1,0 +1,57 @@
+"""Builds the HIP/C++ solver library rustyfem_amd/librustyfem_ffi.so for gfx950.
+
+Host-only translation units are compiled with g++, HIP units with hipcc
+(--offload-arch=gfx950), and everything is linked by hipcc.  Objects go to
+rustyfem_amd/_build/.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "librustyfem_ffi.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+HOST_SOURCES = ["symbolic.cpp", "mf_plan.cpp", "ffi.cpp"]
+HIP_SOURCES = ["mf_gpu.hip", "indef_gpu.hip"]
+
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops", "-Wno-unused-result", "-Wno-pass-failed"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    return err
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    jobs = []
+    for s in HOST_SOURCES:
+        o = os.path.join(OUT, s + ".o")
+        jobs.append((["g++"] + CXXFLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
+    for s in HIP_SOURCES:
+        o = os.path.join(OUT, s + ".o")
+        jobs.append(([HIPCC] + HIPFLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for msg in ex.map(lambda j: _run(j[0]), jobs):
+            if verbose and msg:
+                print(msg, file=sys.stderr)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + [o for _, o in jobs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

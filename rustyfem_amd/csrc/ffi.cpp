@@ -1,0 +1,417 @@
+// The C ABI of ffi/src/lib.rs, served by the MI355X multifrontal solver.
+//
+// build() restates ffi/src/lib.rs:100-229 (validation, uncoupled DOFs,
+// node compaction, adjacency, nested dissection with leaf 16, equation map,
+// entry filter per mode); the numeric factorization and the solves run on
+// the GPU (mf_gpu.hip, indef_gpu.hip).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rustyfem.h"
+#include "gpu.hpp"
+#include "indef.hpp"
+#include "mf.hpp"
+#include "symbolic.hpp"
+
+namespace {
+
+enum class Mode { Pd, Indef, Lu };
+
+struct Options {
+  rf::Precision prec = rf::Precision::FP64;
+  int64_t leaf = 16;  // ffi/src/lib.rs:17
+  int max_refine = 0;
+  double refine_tol = 1e-10;
+};
+
+Options resolve(const rustyfem_options* o) {
+  Options r;
+  if (!o) return r;
+  r.prec = o->precision == 1 ? rf::Precision::MIXED : rf::Precision::FP64;
+  r.leaf = o->leaf > 0 ? o->leaf : 16;
+  r.max_refine = o->max_refine >= 0 ? o->max_refine : (r.prec == rf::Precision::MIXED ? 30 : 0);
+  r.refine_tol = o->refine_tol > 0 ? o->refine_tol : 1e-10;
+  return r;
+}
+
+}  // namespace
+
+struct Handle {
+  Mode mode = Mode::Pd;
+  Options opt;
+  int64_t n = 0;                 // caller DOFs
+  std::vector<int64_t> eq;       // internal equation per caller DOF, -1 = uncoupled
+  std::vector<double> diag;      // caller diagonal (uncoupled DOFs use it)
+  std::vector<uint8_t> coupled;
+  int64_t n_sym = 0;
+  double work = 0.0;
+  int64_t factor_size = 0;
+  bool has_inertia = false;
+  int64_t inertia[3] = {0, 0, 0};
+  int64_t nfronts = 0, nlevels = 0;
+  // pattern: caller entry p -> Lower value index (-1 = dropped by the filter)
+  std::vector<int64_t> indptr, indices, vmap;
+  std::vector<int64_t> colptr, rowidx;
+  std::vector<int64_t> owner;
+  std::unique_ptr<rf::GpuFactor> gpu;
+  std::unique_ptr<rf::IndefGpu> ind;
+};
+
+namespace {
+
+void set_error(const std::string& s) { rf::last_error() = s; }
+
+// ffi/src/lib.rs:100-229 up to the factorization; returns nullptr with *err.
+// With gpu == false only the host-side symbolic work is done.
+Handle* build(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+              const int64_t* indices, const double* data, Mode mode, const Options& opt, int64_t* err,
+              bool gpu = true) {
+  *err = -1;
+  const int64_t nnz = indptr[n];
+  for (size_t i = 0; i < n; ++i)
+    if (loc[i] < 0 || static_cast<size_t>(loc[i]) >= nn) return nullptr;
+  for (int64_t p = 0; p < nnz; ++p)
+    if (indices[p] < 0 || static_cast<size_t>(indices[p]) >= n) return nullptr;
+  auto h = std::make_unique<Handle>();
+  h->mode = mode;
+  h->opt = opt;
+  h->n = static_cast<int64_t>(n);
+  h->coupled.assign(n, 0);
+  h->diag.assign(n, 0.0);
+  for (size_t c = 0; c < n; ++c)
+    for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+      const size_t r = static_cast<size_t>(indices[p]);
+      if (r == c) {
+        h->diag[c] += data[p];
+      } else if (data[p] != 0.0) {
+        h->coupled[c] = 1;
+        h->coupled[r] = 1;
+      }
+    }
+  for (size_t i = 0; i < n; ++i) {
+    if (h->coupled[i]) continue;
+    const double d = h->diag[i];
+    const bool ok = mode == Mode::Pd ? d > 0.0 : (d != 0.0 && std::isfinite(d));
+    if (!ok) {
+      *err = static_cast<int64_t>(i);
+      return nullptr;
+    }
+  }
+  // nodes with a coupled DOF, compact numbering in first-appearance order
+  std::vector<int64_t> nid(nn, -1);
+  std::vector<double> xyz;
+  int64_t nc = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!h->coupled[i]) continue;
+    const int64_t v = loc[i];
+    if (nid[v] < 0) {
+      nid[v] = nc++;
+      xyz.insert(xyz.end(), {coords[3 * v], coords[3 * v + 1], coords[3 * v + 2]});
+    }
+  }
+  std::vector<int64_t> ndof(nc, 0), local(n, 0);
+  for (size_t i = 0; i < n; ++i)
+    if (h->coupled[i]) local[i] = ndof[nid[loc[i]]]++;
+  std::vector<std::vector<int64_t>> adj(nc);
+  for (size_t c = 0; c < n; ++c) {
+    if (!h->coupled[c]) continue;
+    for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+      const size_t r = static_cast<size_t>(indices[p]);
+      if (h->coupled[r] && nid[loc[r]] != nid[loc[c]]) {
+        adj[nid[loc[c]]].push_back(nid[loc[r]]);
+        adj[nid[loc[r]]].push_back(nid[loc[c]]);
+      }
+    }
+  }
+  std::vector<int64_t> aptr(nc + 1, 0), aidx;
+  for (int64_t v = 0; v < nc; ++v) {
+    auto& a = adj[v];
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+    aptr[v + 1] = aptr[v] + static_cast<int64_t>(a.size());
+  }
+  aidx.reserve(aptr[nc]);
+  for (auto& a : adj) {
+    aidx.insert(aidx.end(), a.begin(), a.end());
+    std::vector<int64_t>().swap(a);
+  }
+  rf::Symbolic sym = rf::analyse(xyz.data(), nc, aptr, aidx, ndof, opt.leaf);
+  h->eq.assign(n, -1);
+  for (size_t i = 0; i < n; ++i)
+    if (h->coupled[i]) h->eq[i] = sym.first_dof[nid[loc[i]]] + local[i];
+  h->n_sym = sym.n;
+  h->owner.assign(sym.n, 0);
+  for (size_t i = 0; i < n; ++i)
+    if (h->eq[i] >= 0) h->owner[h->eq[i]] = static_cast<int64_t>(i);
+  sym.pattern(aptr, aidx, ndof, h->colptr, h->rowidx);
+  h->work = sym.work();
+  h->factor_size = sym.factor_size();
+  h->nfronts = static_cast<int64_t>(sym.fronts.size());
+  // entry filter per mode (ffi/src/lib.rs:176-186, 193-202, 216-224): vmap[p] is
+  // the Lower index the caller entry p adds to, or -1
+  h->indptr.assign(indptr, indptr + n + 1);
+  h->indices.assign(indices, indices + nnz);
+  h->vmap.assign(nnz, -1);
+  for (size_t c = 0; c < n; ++c) {
+    if (!h->coupled[c]) continue;
+    const int64_t qc = h->eq[c];
+    for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+      const size_t r = static_cast<size_t>(indices[p]);
+      const int64_t qr = h->eq[r];
+      if (qr < 0) continue;
+      int64_t row, col;
+      if (mode == Mode::Pd) {
+        if (qr < qc) continue;
+        row = qr;
+        col = qc;
+      } else if (mode == Mode::Indef) {
+        if (r < c) continue;
+        row = std::max(qr, qc);
+        col = std::min(qr, qc);
+      } else {
+        // LU: lower triangle entries by (qr, qc); upper ones are marked with an offset
+        row = std::max(qr, qc);
+        col = std::min(qr, qc);
+      }
+      const auto b = h->rowidx.begin() + h->colptr[col], e = h->rowidx.begin() + h->colptr[col + 1];
+      const auto it = std::lower_bound(b, e, row);
+      if (it == e || *it != row) throw std::runtime_error("internal: entry missing in the pattern");
+      int64_t vi = it - h->rowidx.begin();
+      if (mode == Mode::Lu && qr < qc) vi += static_cast<int64_t>(h->rowidx.size());
+      h->vmap[p] = vi;
+    }
+  }
+  if (!gpu) return h.release();
+  if (mode == Mode::Pd) {
+    rf::Plan plan = rf::build_plan(sym, h->colptr, h->rowidx);
+    h->nlevels = static_cast<int64_t>(plan.levels.size());
+    h->gpu = rf::make_gpu_factor(std::move(plan), h->colptr, h->rowidx, opt.prec);
+    h->gpu->set_caller_map(h->eq, h->diag);
+  } else {
+    h->ind = rf::make_indef_gpu(std::move(sym), h->colptr, h->rowidx, mode == Mode::Lu);
+    h->ind->set_caller_map(h->eq, h->diag);
+  }
+  return h.release();
+}
+
+// Lower values of the handle's pattern from caller data (the filter of the mode).
+std::vector<double> lower_values(const Handle* h, const double* data) {
+  const size_t nl = h->rowidx.size();
+  std::vector<double> v(h->mode == Mode::Lu ? 2 * nl : nl, 0.0);
+  for (size_t p = 0; p < h->vmap.size(); ++p)
+    if (h->vmap[p] >= 0) v[h->vmap[p]] += data[p];
+  return v;
+}
+
+// numeric factorization; returns -1 or the caller DOF of the failing pivot
+int64_t numeric(Handle* h, const double* data) {
+  if (h->mode == Mode::Pd) {
+    int64_t e;
+    if (data) {
+      const std::vector<double> v = lower_values(h, data);
+      e = h->gpu->factor(v.data(), false);
+    } else {
+      e = h->gpu->factor(nullptr, true);
+    }
+    return e < 0 ? -1 : h->owner[e];
+  }
+  const std::vector<double> v = lower_values(h, data);
+  const int64_t e = h->ind->factor(v.data());
+  if (e >= 0) return h->owner[e];
+  if (h->mode == Mode::Indef) {
+    int64_t ine[3];
+    h->ind->inertia(ine);
+    for (int64_t i = 0; i < h->n; ++i)
+      if (!h->coupled[i]) (h->diag[i] < 0.0 ? ine[0] : ine[2]) += 1;
+    std::memcpy(h->inertia, ine, sizeof ine);
+    h->has_inertia = true;
+  }
+  h->factor_size = h->ind->factor_size();
+  return -1;
+}
+
+Handle* factor_mode(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                    const int64_t* indices, const double* data, Mode mode, const Options& opt, double* d_out,
+                    int64_t* err_dof) {
+  *err_dof = -1;
+  set_error("");
+  Handle* h = nullptr;
+  try {
+    int64_t e = -1;
+    h = build(n, nn, loc, coords, indptr, indices, data, mode, opt, &e);
+    if (!h) {
+      *err_dof = e;
+      return nullptr;
+    }
+    const int64_t q = numeric(h, data);
+    if (q >= 0) {
+      *err_dof = q;
+      delete h;
+      return nullptr;
+    }
+    if (mode == Mode::Pd) {
+      h->has_inertia = true;
+      h->inertia[0] = 0;
+      h->inertia[1] = 0;
+      h->inertia[2] = static_cast<int64_t>(n);
+      if (d_out) {
+        std::vector<double> dq(h->n_sym);
+        h->gpu->pivots(dq.data());
+        for (size_t i = 0; i < n; ++i) d_out[i] = h->coupled[i] ? dq[h->eq[i]] : h->diag[i];
+      }
+    }
+    return h;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  delete h;
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+Handle* rustyfem_factor(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                        const int64_t* indices, const double* data, double* d_out, int64_t* err_dof) {
+  return factor_mode(n, nn, loc, coords, indptr, indices, data, Mode::Pd, Options{}, d_out, err_dof);
+}
+
+Handle* rustyfem_factor_ex(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                           const int64_t* indices, const double* data, const rustyfem_options* opt, double* d_out,
+                           int64_t* err_dof) {
+  return factor_mode(n, nn, loc, coords, indptr, indices, data, Mode::Pd, resolve(opt), d_out, err_dof);
+}
+
+Handle* rustyfem_factor_indef(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                              const int64_t* indices, const double* data, int64_t* inertia_out, int64_t* err_dof) {
+  Handle* h = factor_mode(n, nn, loc, coords, indptr, indices, data, Mode::Indef, Options{}, nullptr, err_dof);
+  if (h && inertia_out) std::memcpy(inertia_out, h->inertia, 3 * sizeof(int64_t));
+  return h;
+}
+
+Handle* rustyfem_factor_unsym(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                              const int64_t* indices, const double* data, int64_t* err_dof) {
+  return factor_mode(n, nn, loc, coords, indptr, indices, data, Mode::Lu, Options{}, nullptr, err_dof);
+}
+
+int32_t rustyfem_inertia(const Handle* h, int64_t* out) {
+  if (!h->has_inertia) return -1;
+  std::memcpy(out, h->inertia, 3 * sizeof(int64_t));
+  return 0;
+}
+
+int32_t rustyfem_solve(const Handle* h, size_t nrhs, double* b) {
+  if (h->n == 0 || nrhs == 0) return 0;
+  try {
+    if (h->mode == Mode::Pd)
+      h->gpu->solve(b, static_cast<int>(nrhs), false, h->opt.refine_tol, h->opt.max_refine);
+    else
+      h->ind->solve(b, static_cast<int>(nrhs));
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_info(const Handle* h, double* out) {
+  out[0] = h->work;
+  out[1] = static_cast<double>(h->factor_size);
+}
+
+void rustyfem_order(const Handle* h, int64_t* out) {
+  int64_t next = h->n_sym;
+  for (int64_t i = 0; i < h->n; ++i) out[i] = h->eq[i] < 0 ? next++ : h->eq[i];
+}
+
+void rustyfem_free(Handle* h) { delete h; }
+
+int32_t rustyfem_refactor(Handle* h, const double* data, int64_t* err_dof) {
+  *err_dof = -1;
+  try {
+    if (!data && h->mode != Mode::Pd) throw std::runtime_error("refactor without data needs a positive definite handle");
+    const int64_t q = numeric(h, data);
+    if (q >= 0) {
+      *err_dof = q;
+      return 1;
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
+  if (h->n == 0 || nrhs == 0) return 0;
+  try {
+    if (h->mode != Mode::Pd) throw std::runtime_error("device solves need a positive definite handle");
+    h->gpu->solve(d_b, static_cast<int>(nrhs), true, h->opt.refine_tol, h->opt.max_refine);
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_stats(const Handle* h, double* out, size_t len) {
+  double v[10] = {0};
+  if (h->gpu) {
+    const rf::Timings& t = h->gpu->timings();
+    v[0] = t.factor_ms;
+    v[1] = t.solve_ms;
+    v[2] = t.refine_ms;
+    v[3] = t.refine_steps;
+    v[4] = t.rel_residual;
+    v[5] = h->gpu->flops();
+    v[6] = static_cast<double>(h->gpu->device_bytes());
+  }
+  v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
+  v[8] = static_cast<double>(h->nfronts);
+  v[9] = static_cast<double>(h->nlevels);
+  for (size_t i = 0; i < len && i < 10; ++i) out[i] = v[i];
+}
+
+const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
+
+int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                         const int64_t* indices, const double* data, int32_t leaf, int64_t* order_out,
+                         double* info_out, int64_t* err_dof) {
+  *err_dof = -1;
+  set_error("");
+  try {
+    Options opt;
+    opt.leaf = leaf > 0 ? leaf : 16;
+    int64_t e = -1;
+    std::unique_ptr<Handle> h(build(n, nn, loc, coords, indptr, indices, data, Mode::Pd, opt, &e, false));
+    if (!h) {
+      *err_dof = e;
+      return 1;
+    }
+    if (order_out) rustyfem_order(h.get(), order_out);
+    if (info_out) rustyfem_info(h.get(), info_out);
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Device side of the multifrontal solver: owns the HBM buffers of one
+// factorization and runs the level-batched HIP kernels on its own stream.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mf.hpp"
+
+namespace rf {
+
+enum class Precision : int32_t { FP64 = 0, MIXED = 1 };
+
+struct Timings {
+  double factor_ms = 0, solve_ms = 0, refine_ms = 0;
+  int32_t refine_steps = 0;
+  double rel_residual = -1;
+};
+
+class GpuFactor {
+ public:
+  virtual ~GpuFactor() = default;
+  // Numeric factorization from Lower values (internal numbering, host or device
+  // pointer).  Returns -1, or the first equation whose pivot fails the test.
+  virtual int64_t factor(const double* lower_vals, bool on_device) = 0;
+  // Caller numbering: eq[i] is the internal equation of caller DOF i (-1 for an
+  // uncoupled DOF, solved as b / diag with its entry in iso_diag).
+  virtual void set_caller_map(const std::vector<int64_t>& eq, const std::vector<double>& iso_diag) = 0;
+  // In-place solve of nrhs columns in caller numbering (column-major
+  // n_caller x nrhs, host or device pointer).  With max_steps > 0 the solve
+  // is followed by FP64 iterative refinement against the FP64 matrix until
+  // ||b - A x|| <= tol ||b|| for every column or max_steps corrections.
+  virtual void solve(double* b, int nrhs, bool on_device, double tol, int max_steps) = 0;
+  virtual void pivots(double* d_host) = 0;
+  virtual const Timings& timings() const = 0;
+  virtual double flops() const = 0;
+  virtual size_t device_bytes() const = 0;
+  virtual void sync() = 0;
+  // kernel-level timing of the Schur-update GEMMs of the last factorization
+  virtual double gemm_ms() const = 0;
+};
+
+// lower: Lower pattern (internal numbering), used to build the refinement matrix
+std::unique_ptr<GpuFactor> make_gpu_factor(Plan plan, const std::vector<int64_t>& colptr,
+                                           const std::vector<int64_t>& rowidx, Precision prec);
+
+// last HIP error message of this thread (empty if none)
+std::string& last_error();
+
+}  // namespace rf

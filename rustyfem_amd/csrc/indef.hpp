@@ -1,0 +1,27 @@
+// Indefinite LDL^T (1x1/2x2 pivots, delayed pivots) and unsymmetric LU on the
+// GPU, on the same nested-dissection tree as the positive definite solver.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "symbolic.hpp"
+
+namespace rf {
+
+class IndefGpu {
+ public:
+  virtual ~IndefGpu() = default;
+  virtual void set_caller_map(const std::vector<int64_t>& eq, const std::vector<double>& iso_diag) = 0;
+  // lower values of the pattern (LU: lower values followed by the transposed
+  // strictly-upper values on the same pattern).  Returns -1 or the singular equation.
+  virtual int64_t factor(const double* vals) = 0;
+  virtual void solve(double* b, int nrhs) = 0;  // caller numbering, host pointer
+  virtual void inertia(int64_t out[3]) const = 0;
+  virtual int64_t factor_size() const = 0;
+};
+
+std::unique_ptr<IndefGpu> make_indef_gpu(Symbolic sym, const std::vector<int64_t>& colptr,
+                                         const std::vector<int64_t>& rowidx, bool lu);
+
+}  // namespace rf

@@ -1,0 +1,86 @@
+// Multifrontal LDL^T on one MI355X: plan (host, integer work) and device driver.
+//
+// Layout in HBM: every front f owns a dense m x m column-major block (leading
+// dimension m = k + u) in one arena; its first k columns become the factor
+// (L11 strictly lower, L21 below), its trailing u x u lower block is the update
+// matrix its parent extend-adds.  D lives in a separate FP64 array indexed by
+// equation.  Fronts are processed level by level (a front's level is one more
+// than its highest child), so every launch covers all fronts of a level.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "symbolic.hpp"
+
+namespace rf {
+
+constexpr int NB = 64;        // pivot block width (factor and solve)
+constexpr int TR = 256;       // rows per TRSM / solve-update tile
+constexpr int GT_M = 128;     // GEMM tile rows
+constexpr int GT_N = 64;      // GEMM tile columns
+constexpr int EA_T = 64;      // extend-add tile edge
+constexpr int ZC = 64;        // columns per zero tile
+
+struct FrontDesc {
+  int64_t off;     // element offset of the front in the arena
+  int64_t toff;    // element offset of the front's vector in the solve buffers
+  int32_t m, k, u, start;
+  int32_t parent;
+  int32_t cmap_off;  // offset of this front's child->parent row map (length u), -1 for a root
+};
+
+struct ZeroTile { int32_t front, c0; };
+struct ScatterTile { int64_t p0, p1; };
+struct EaTile { int32_t child, x0, y0, pad; };
+struct GemmTile { int32_t front, r0, c0, cend, kdim, pad; };
+struct DiagTask { int32_t front, j0, jb, slot; int32_t tile0, ntiles, pad0, pad1; };
+struct TrsmTile { int32_t front, j0, jb, r0, slot, local; };
+
+struct Range { int64_t begin = 0, count = 0; };
+
+struct StepPlan {
+  Range upd;    // left-looking update of the block column (GemmTile)
+  Range diag;   // diagonal block tasks (DiagTask)
+  Range trsm;   // row tiles below the block (TrsmTile)
+};
+
+struct LevelPlan {
+  std::vector<int32_t> fronts;
+  Range zero;      // ZeroTile
+  Range scatter;   // ScatterTile
+  Range ea[2];     // EaTile per child slot
+  std::vector<StepPlan> steps;
+  Range syrk;      // GemmTile (lower tiles of the update matrices)
+  Range fronts_r;  // range in the level-ordered front list (solve kernels)
+  Range child_r[2];  // fronts of this level that are child slot s of their parent... (unused for now)
+};
+
+struct Plan {
+  int64_t n = 0, nf = 0;
+  std::vector<FrontDesc> fd;
+  std::vector<int32_t> level_of;
+  std::vector<LevelPlan> levels;
+  std::vector<int32_t> level_fronts;  // fronts grouped by level (solve kernels)
+  std::vector<int32_t> cmap;          // child row -> parent local index
+  std::vector<int64_t> amap;          // Lower entry -> arena offset
+  std::vector<ZeroTile> zero;
+  std::vector<ScatterTile> scatter;
+  std::vector<EaTile> ea;
+  std::vector<GemmTile> gemm;
+  std::vector<DiagTask> diag;
+  std::vector<TrsmTile> trsm;
+  std::vector<int32_t> child_slot_fronts[2];  // fronts that are child slot s (per level ranges below)
+  std::vector<Range> child_slot_range[2];     // per level
+  int64_t arena_elems = 0;
+  int64_t tvec_elems = 0;  // sum of m
+  int32_t max_slots = 0;   // most diag tasks in one step
+  int32_t max_tiles_per_task = 0;
+  int64_t max_partials = 0;  // most trsm tiles in one step
+  double flops = 0.0;        // sum k^3/3 + u k^2 + u^2 k
+};
+
+// Builds the plan from the symbolic analysis and the Lower pattern (internal numbering).
+Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const std::vector<int64_t>& rowidx);
+
+}  // namespace rf

@@ -1,0 +1,190 @@
+"""ctypes binding of librustyfem_ffi.so -- the drop-in C ABI of RustyFEM's ffi/
+library (ffi/src/lib.rs:64-407) served by HIP kernels on an MI355X.
+
+The class mirrors what a ctypes caller of the reference library does: the
+same eight ``rustyfem_*`` functions with the same argument meaning, error
+codes and in-place solves, plus the additive ``_ex``/``refactor``/``stats``
+entry points.  The library is built by ``rustyfem_amd.build``; importing the
+binding without it raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librustyfem_ffi.so")
+
+EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
+           "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
+           "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse"]
+
+_lib = None
+
+
+class Options(C.Structure):
+    """rustyfem_options (include/rustyfem.h)."""
+    _fields_ = [("precision", C.c_int32), ("leaf", C.c_int32), ("max_refine", C.c_int32), ("reserved", C.c_int32),
+                ("refine_tol", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rustyfem_amd.build` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i64p, f64p = C.c_void_p, C.c_size_t, C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        fac = [sz, sz, vp, vp, vp, vp, vp]
+        L.rustyfem_factor.restype = vp
+        L.rustyfem_factor.argtypes = fac + [vp, i64p]
+        L.rustyfem_factor_ex.restype = vp
+        L.rustyfem_factor_ex.argtypes = fac + [C.POINTER(Options), vp, i64p]
+        L.rustyfem_factor_indef.restype = vp
+        L.rustyfem_factor_indef.argtypes = fac + [vp, i64p]
+        L.rustyfem_factor_unsym.restype = vp
+        L.rustyfem_factor_unsym.argtypes = fac + [i64p]
+        L.rustyfem_inertia.restype = C.c_int32
+        L.rustyfem_inertia.argtypes = [vp, vp]
+        L.rustyfem_solve.restype = C.c_int32
+        L.rustyfem_solve.argtypes = [vp, sz, vp]
+        L.rustyfem_solve_device.restype = C.c_int32
+        L.rustyfem_solve_device.argtypes = [vp, sz, vp]
+        L.rustyfem_info.restype = None
+        L.rustyfem_info.argtypes = [vp, vp]
+        L.rustyfem_order.restype = None
+        L.rustyfem_order.argtypes = [vp, vp]
+        L.rustyfem_free.restype = None
+        L.rustyfem_free.argtypes = [vp]
+        L.rustyfem_refactor.restype = C.c_int32
+        L.rustyfem_refactor.argtypes = [vp, vp, i64p]
+        L.rustyfem_stats.restype = None
+        L.rustyfem_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_last_error.restype = C.c_char_p
+        L.rustyfem_last_error.argtypes = []
+        L.rustyfem_analyse.restype = C.c_int32
+        L.rustyfem_analyse.argtypes = fac + [C.c_int32, vp, vp, i64p]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().rustyfem_last_error().decode()
+
+
+def analyse(p, leaf=16):
+    """Host-only symbolic analysis (rustyfem_analyse): (order, info, err_dof)."""
+    keep, args = _arrays(p)
+    order = np.zeros(max(int(p.n), 1), dtype=np.int64)
+    info = np.zeros(2)
+    err = C.c_int64(0)
+    rc = lib().rustyfem_analyse(*args, int(leaf), order.ctypes.data, info.ctypes.data, C.byref(err))
+    if rc < 0:
+        raise RuntimeError("rustyfem_analyse failed: " + last_error())
+    return order[:int(p.n)], info, int(err.value)
+
+
+def _arrays(p):
+    keep = (np.ascontiguousarray(p.loc, dtype=np.int64), np.ascontiguousarray(np.asarray(p.coords).reshape(-1),
+                                                                              dtype=np.float64),
+            np.ascontiguousarray(p.indptr, dtype=np.int64), np.ascontiguousarray(p.indices, dtype=np.int64),
+            np.ascontiguousarray(p.data, dtype=np.float64))
+    return keep, [int(p.n), int(p.nn)] + [a.ctypes.data for a in keep]
+
+
+class Handle:
+    """A factorization held by the library (ffi/src/lib.rs:44-55)."""
+
+    def __init__(self, ptr, n, indptr=None, indices=None):
+        self._h = ptr
+        self.n = n
+        self.indptr = indptr
+        self.indices = indices
+
+    @classmethod
+    def factor(cls, p, want_d=False, precision=None, leaf=0, max_refine=-1, refine_tol=0.0):
+        """rustyfem_factor (precision None) or rustyfem_factor_ex.  Returns (handle|None, err_dof, d)."""
+        keep, args = _arrays(p)
+        err = C.c_int64(0)
+        d = np.zeros(max(int(p.n), 1)) if want_d else None
+        dptr = d.ctypes.data if want_d else None
+        if precision is None:
+            h = lib().rustyfem_factor(*args, dptr, C.byref(err))
+        else:
+            o = Options(1 if precision in (1, "mixed") else 0, int(leaf), int(max_refine), 0, float(refine_tol))
+            h = lib().rustyfem_factor_ex(*args, C.byref(o), dptr, C.byref(err))
+        hd = cls(h, int(p.n), keep[2], keep[3]) if h else None
+        return hd, int(err.value), (d[:int(p.n)] if want_d else None)
+
+    @classmethod
+    def factor_indef(cls, p):
+        keep, args = _arrays(p)
+        err = C.c_int64(0)
+        ine = np.zeros(3, dtype=np.int64)
+        h = lib().rustyfem_factor_indef(*args, ine.ctypes.data, C.byref(err))
+        return (cls(h, int(p.n)) if h else None), int(err.value), ine
+
+    @classmethod
+    def factor_unsym(cls, p):
+        keep, args = _arrays(p)
+        err = C.c_int64(0)
+        h = lib().rustyfem_factor_unsym(*args, C.byref(err))
+        return (cls(h, int(p.n)) if h else None), int(err.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().rustyfem_free(self._h)
+            self._h = None
+
+    def free(self):
+        self.__del__()
+
+    def solve(self, b):
+        b = np.asarray(b, dtype=np.float64)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        flat = np.ascontiguousarray(b.reshape(-1, order="F"))
+        rc = lib().rustyfem_solve(self._h, nrhs, flat.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("rustyfem_solve failed: " + last_error())
+        return flat.reshape(b.shape, order="F")
+
+    def solve_device(self, ptr, nrhs):
+        rc = lib().rustyfem_solve_device(self._h, int(nrhs), ptr)
+        if rc != 0:
+            raise RuntimeError("rustyfem_solve_device failed: " + last_error())
+
+    def refactor(self, data=None):
+        err = C.c_int64(0)
+        if data is None:
+            rc = lib().rustyfem_refactor(self._h, None, C.byref(err))
+        else:
+            d = np.ascontiguousarray(data, dtype=np.float64)
+            rc = lib().rustyfem_refactor(self._h, d.ctypes.data, C.byref(err))
+        if rc < 0:
+            raise RuntimeError("rustyfem_refactor failed: " + last_error())
+        return int(err.value) if rc == 1 else -1
+
+    def inertia(self):
+        o = np.zeros(3, dtype=np.int64)
+        rc = lib().rustyfem_inertia(self._h, o.ctypes.data)
+        return o if rc == 0 else None
+
+    def info(self):
+        o = np.zeros(2)
+        lib().rustyfem_info(self._h, o.ctypes.data)
+        return o
+
+    def order(self):
+        o = np.zeros(max(self.n, 1), dtype=np.int64)
+        lib().rustyfem_order(self._h, o.ctypes.data)
+        return o[:self.n]
+
+    def stats(self):
+        o = np.zeros(10)
+        lib().rustyfem_stats(self._h, o.ctypes.data, 10)
+        keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
+                "precision", "fronts", "levels"]
+        return dict(zip(keys, o.tolist()))

@@ -1,0 +1,93 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol that
+include/rustyfem.h declares, and its host-side symbolic phase (nested
+dissection, equation map, entry filter, flop and fill counts) is identical to
+the oracle's."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rustyfem_amd import meshes
+from tests._problems import chain_problem, dense_problem, laplacian3d
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def rf():
+    from rustyfem_amd import ffi
+    ffi.lib()
+    return ffi
+
+
+def test_exports_match_header(rf):
+    hdr = open(os.path.join(ROOT, "include", "rustyfem.h")).read()
+    declared = set(re.findall(r"\b(rustyfem_[a-z_]+)\s*\(", hdr))
+    assert {"rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
+            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free"} <= declared
+    lib = rf.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(rf.EXPORTS)
+
+
+@pytest.mark.parametrize("case", ["cantilever", "hex7", "hex_hetero", "plate", "laplace", "random"])
+def test_host_symbolic_identical_to_oracle(rf, ora, case):
+    if case == "cantilever":
+        p = meshes.cantilever_cps4()
+    elif case == "hex7":
+        p = meshes.hex_block(7)
+    elif case == "hex_hetero":
+        p = meshes.hetero_block(6)
+    elif case == "plate":
+        p = meshes.plate_cpe4(60, 6, 1e11, 0.3)
+    elif case == "laplace":
+        p = laplacian3d(9, 0.0)
+    else:
+        rng = np.random.default_rng(7)
+        nn = 80
+        coords = rng.uniform(0, 1, (nn, 3))
+        a = np.zeros((2 * nn, 2 * nn))
+        for i in range(nn):
+            for j in range(i + 1, nn):
+                if np.linalg.norm(coords[i] - coords[j]) < 0.3:
+                    a[2 * i:2 * i + 2, 2 * j:2 * j + 2] = rng.standard_normal((2, 2))
+                    a[2 * j:2 * j + 2, 2 * i:2 * i + 2] = a[2 * i:2 * i + 2, 2 * j:2 * j + 2].T
+        a += np.eye(2 * nn) * 50
+        p = dense_problem(a, coords=coords, loc=np.repeat(np.arange(nn), 2))
+    order, info, err = rf.analyse(p)
+    ho, eo, _ = ora.Handle.factor(p)
+    assert err == -1
+    assert ho is not None
+    assert np.array_equal(order, ho.order())
+    assert np.array_equal(info, ho.info())
+
+
+def test_host_rejects_invalid_and_uncoupled(rf, ora):
+    a = np.array([[2.0, -1.0, 0.0], [-1.0, 2.0, 0.0], [0.0, 0.0, -3.0]])
+    p = dense_problem(a)
+    order, info, err = rf.analyse(p)
+    assert err == 2  # uncoupled DOF with a non-positive diagonal (ffi/src/lib.rs:122-126)
+    p.loc = np.array([0, 1, 9], dtype=np.int64)
+    order, info, err = rf.analyse(p)
+    assert err == -1
+    q = chain_problem(30, lambda c: 2.0, lambda r, c: -1.0)
+    order, info, err = rf.analyse(q)
+    ho, _, _ = ora.Handle.factor(q)
+    assert np.array_equal(order, ho.order())
+
+
+@pytest.mark.parametrize("leaf", [1, 8, 48])
+def test_leaf_option(rf, ora, leaf):
+    p = meshes.hex_block(5)
+    order, info, err = rf.analyse(p, leaf=leaf)
+    ora.set_ffi_leaf = None
+    lib = ora.lib()
+    lib.ora_set_ffi_leaf(leaf)
+    try:
+        ho, _, _ = ora.Handle.factor(p)
+        assert np.array_equal(order, ho.order())
+        assert np.array_equal(info, ho.info())
+    finally:
+        lib.ora_set_ffi_leaf(16)
