@@ -95,7 +95,10 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
 /* Statistics of the last factorization / solve (additive):
  * out[0] factor ms, [1] solve ms, [2] refined-solve ms, [3] refinement steps,
  * [4] relative residual after refinement (-1 if not refined), [5] flops,
- * [6] device bytes held, [7] precision, [8] number of fronts, [9] levels. */
+ * [6] device bytes held, [7] precision, [8] number of fronts, [9] levels,
+ * [10] summed duration (ms) of the Schur-update GEMM launches, [11] their
+ * count and [12] their algorithmic flops -- [10] and [11] are filled when the
+ * environment variable RUSTYFEM_KERNEL_TIMING=1 was set at factorization. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */

@@ -6,6 +6,7 @@
 // the GPU (mf_gpu.hip, indef_gpu.hip).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -193,6 +194,8 @@ Handle* build(size_t n, size_t nn, const int64_t* loc, const double* coords, con
     h->nlevels = static_cast<int64_t>(plan.levels.size());
     h->gpu = rf::make_gpu_factor(std::move(plan), h->colptr, h->rowidx, opt.prec);
     h->gpu->set_caller_map(h->eq, h->diag);
+    const char* kt = std::getenv("RUSTYFEM_KERNEL_TIMING");
+    h->gpu->set_kernel_timing(kt && *kt && *kt != '0');
   } else {
     h->ind = rf::make_indef_gpu(std::move(sym), h->colptr, h->rowidx, mode == Mode::Lu);
     h->ind->set_caller_map(h->eq, h->diag);
@@ -370,7 +373,7 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
 }
 
 void rustyfem_stats(const Handle* h, double* out, size_t len) {
-  double v[10] = {0};
+  double v[13] = {0};
   if (h->gpu) {
     const rf::Timings& t = h->gpu->timings();
     v[0] = t.factor_ms;
@@ -380,11 +383,14 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
     v[4] = t.rel_residual;
     v[5] = h->gpu->flops();
     v[6] = static_cast<double>(h->gpu->device_bytes());
+    v[10] = t.gemm_ms;
+    v[11] = t.gemm_launches;
+    v[12] = t.gemm_flops;
   }
   v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
   v[8] = static_cast<double>(h->nfronts);
   v[9] = static_cast<double>(h->nlevels);
-  for (size_t i = 0; i < len && i < 10; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }

@@ -16,6 +16,8 @@ struct Timings {
   double factor_ms = 0, solve_ms = 0, refine_ms = 0;
   int32_t refine_steps = 0;
   double rel_residual = -1;
+  // per-launch HIP-event timing of the Schur-update GEMM kernel (when enabled)
+  double gemm_ms = 0, gemm_launches = 0, gemm_flops = 0;
 };
 
 class GpuFactor {
@@ -37,8 +39,8 @@ class GpuFactor {
   virtual double flops() const = 0;
   virtual size_t device_bytes() const = 0;
   virtual void sync() = 0;
-  // kernel-level timing of the Schur-update GEMMs of the last factorization
-  virtual double gemm_ms() const = 0;
+  // bracket every Schur-update GEMM launch with HIP events during factor()
+  virtual void set_kernel_timing(bool on) = 0;
 };
 
 // lower: Lower pattern (internal numbering), used to build the refinement matrix

@@ -40,6 +40,7 @@ struct TrsmTile { int32_t front, j0, jb, r0, slot, local; };
 struct Range { int64_t begin = 0, count = 0; };
 
 struct StepPlan {
+  double upd_flops = 0;  // algorithmic flops of the update GEMM: 2 (m - j0) jb j0 per front
   Range upd;    // left-looking update of the block column (GemmTile)
   Range diag;   // diagonal block tasks (DiagTask)
   Range trsm;   // row tiles below the block (TrsmTile)
@@ -52,6 +53,7 @@ struct LevelPlan {
   Range ea[2];     // EaTile per child slot
   std::vector<StepPlan> steps;
   Range syrk;      // GemmTile (lower tiles of the update matrices)
+  double syrk_flops = 0;  // algorithmic flops: u (u + 1) k per front (lower triangle)
   Range fronts_r;  // range in the level-ordered front list (solve kernels)
   Range child_r[2];  // fronts of this level that are child slot s of their parent... (unused for now)
 };

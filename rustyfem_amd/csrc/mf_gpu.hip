@@ -627,6 +627,7 @@ class Impl final : public GpuFactor {
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
+    for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     (void)hipEventDestroy(e0_);
     (void)hipEventDestroy(e1_);
     (void)hipStreamDestroy(s_);
@@ -639,19 +640,20 @@ class Impl final : public GpuFactor {
     HIPCHK(hipEventRecord(e0_, s_));
     launch(k_gather, (n_ + 255) / 256, 256, d_dpos_, d_lval_, d_adiag_, n_);
     launch(k_gather, (nfull_ + 255) / 256, 256, d_gmap_, d_lval_, d_av_, nfull_);
-    gemm_ms_ = 0.0;
+    nev_ = 0;
+    tm_.gemm_flops = 0;
     for (const LevelPlan& lp : P_.levels) {
       launch(k_zero<T>, lp.zero.count, 256, d_fd_, d_zero_ + lp.zero.begin, d_arena_);
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
       for (const StepPlan& sp : lp.steps) {
-        launch(k_gemm<T>, sp.upd.count, 256, d_fd_, d_gemm_ + sp.upd.begin, d_arena_, d_dpiv_);
+        gemm(sp.upd, sp.upd_flops);
         launch(k_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_dpiv_, d_adiag_, d_l11_,
                d_err_);
         launch(k_trsm<T>, sp.trsm.count, 256, d_fd_, d_trsm_ + sp.trsm.begin, d_arena_, d_dpiv_, d_l11_);
       }
-      launch(k_gemm<T>, lp.syrk.count, 256, d_fd_, d_gemm_ + lp.syrk.begin, d_arena_, d_dpiv_);
+      gemm(lp.syrk, lp.syrk_flops);
     }
     HIPCHK(hipEventRecord(e1_, s_));
     unsigned long long err = 0;
@@ -660,6 +662,15 @@ class Impl final : public GpuFactor {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
     tm_.factor_ms = ms;
+    if (ktime_) {
+      tm_.gemm_ms = 0;
+      for (int64_t i = 0; i < nev_; ++i) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, evs_[2 * i], evs_[2 * i + 1]));
+        tm_.gemm_ms += t;
+      }
+      tm_.gemm_launches = static_cast<double>(nev_);
+    }
     return err == ULLONG_MAX ? -1 : static_cast<int64_t>(err);
   }
 
@@ -709,7 +720,7 @@ class Impl final : public GpuFactor {
   double flops() const override { return P_.flops; }
   size_t device_bytes() const override { return bytes_; }
   void sync() override { HIPCHK(hipStreamSynchronize(s_)); }
-  double gemm_ms() const override { return gemm_ms_; }
+  void set_kernel_timing(bool on) override { ktime_ = on; }
 
  private:
   template <typename K, typename... A>
@@ -717,6 +728,27 @@ class Impl final : public GpuFactor {
     if (blocks <= 0) return;
     hipLaunchKernelGGL(kernel, dim3(grid(blocks)), dim3(threads), 0, s_, args...);
     HIPCHK(hipGetLastError());
+  }
+
+  // one launch of the Schur-update GEMM over a tile range
+  void gemm(const Range& r, double flops) {
+    if (r.count <= 0) return;
+    if (ktime_) {
+      if (nev_ >= static_cast<int64_t>(evs_.size()) / 2) {
+        for (int i = 0; i < 2; ++i) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          evs_.push_back(e);
+        }
+      }
+      HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
+    }
+    launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    tm_.gemm_flops += flops;
+    if (ktime_) {
+      HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
+      ++nev_;
+    }
   }
 
   void ensure_vectors(int nrhs) {
@@ -809,7 +841,9 @@ class Impl final : public GpuFactor {
   int64_t n_ = 0, nlower_ = 0, nfull_ = 0;
   size_t bytes_ = 0;
   Timings tm_;
-  double gemm_ms_ = 0.0;
+  bool ktime_ = false;
+  std::vector<hipEvent_t> evs_;
+  int64_t nev_ = 0;
   int cap_rhs_ = 0;
   FrontDesc* d_fd_ = nullptr;
   int32_t* d_cmap_ = nullptr;

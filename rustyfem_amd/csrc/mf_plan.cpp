@@ -123,6 +123,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
           if (d.k <= j0) continue;
           const int32_t j1 = std::min(j0 + NB, d.k);
           for (int32_t r0 = j0; r0 < d.m; r0 += GT_M) P.gemm.push_back({f, r0, j0, j1, j0, 0});
+          sp.upd_flops += 2.0 * (d.m - j0) * static_cast<double>(j1 - j0) * j0;
         }
       sp.upd = take(b, static_cast<int64_t>(P.gemm.size()));
       const int64_t db = static_cast<int64_t>(P.diag.size());
@@ -150,6 +151,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     for (int32_t f : lp.fronts) {
       const FrontDesc& d = P.fd[f];
       if (d.u == 0 || d.k == 0) continue;
+      lp.syrk_flops += static_cast<double>(d.u) * (d.u + 1.0) * d.k;
       for (int32_t r0 = d.k; r0 < d.m; r0 += GT_M)
         for (int32_t c0 = d.k; c0 < d.m && c0 <= r0 + GT_M - 1; c0 += GT_N) P.gemm.push_back({f, r0, c0, d.m, d.k, 0});
     }

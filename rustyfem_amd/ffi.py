@@ -145,7 +145,7 @@ class Handle:
     def solve(self, b):
         b = np.asarray(b, dtype=np.float64)
         nrhs = 1 if b.ndim == 1 else b.shape[1]
-        flat = np.ascontiguousarray(b.reshape(-1, order="F"))
+        flat = np.array(b.reshape(-1, order="F"), dtype=np.float64, copy=True)
         rc = lib().rustyfem_solve(self._h, nrhs, flat.ctypes.data)
         if rc != 0:
             raise RuntimeError("rustyfem_solve failed: " + last_error())
@@ -183,8 +183,8 @@ class Handle:
         return o[:self.n]
 
     def stats(self):
-        o = np.zeros(10)
-        lib().rustyfem_stats(self._h, o.ctypes.data, 10)
+        o = np.zeros(13)
+        lib().rustyfem_stats(self._h, o.ctypes.data, 13)
         keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
-                "precision", "fronts", "levels"]
+                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops"]
         return dict(zip(keys, o.tolist()))
