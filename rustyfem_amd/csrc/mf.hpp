@@ -6,6 +6,12 @@
 // matrix its parent extend-adds.  D lives in a separate FP64 array indexed by
 // equation.  Fronts are processed level by level (a front's level is one more
 // than its highest child), so every launch covers all fronts of a level.
+//
+// Blocking of the k pivot columns of a front: outer blocks of NBO columns,
+// each factored as inner blocks of NB columns (left-looking inside the outer
+// block), followed by a right-looking update of the remaining pivot columns
+// with K = NBO; the update matrix receives one Schur update with K = k at the
+// end.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -15,10 +21,11 @@
 
 namespace rf {
 
-constexpr int NB = 64;        // pivot block width (factor and solve)
-constexpr int TR = 256;       // rows per TRSM / solve-update tile
+constexpr int NB = 64;        // inner pivot block (diagonal factor, TRSM, solve)
+constexpr int NBO = 256;      // outer pivot block (right-looking updates)
+constexpr int TRS = 64;       // rows per TRSM tile
+constexpr int SR = 256;       // rows per solve tile
 constexpr int GT_M = 128;     // GEMM tile rows
-constexpr int GT_N = 64;      // GEMM tile columns
 constexpr int EA_T = 64;      // extend-add tile edge
 constexpr int ZC = 64;        // columns per zero tile
 
@@ -33,17 +40,24 @@ struct FrontDesc {
 struct ZeroTile { int32_t front, c0; };
 struct ScatterTile { int64_t p0, p1; };
 struct EaTile { int32_t child, x0, y0, pad; };
-struct GemmTile { int32_t front, r0, c0, cend, kdim, pad; };
+// C[r0:r0+128, c0:min(c0+TN, cend)) -= L[r0:, k0:k1] D[k0:k1] L[c0:, k0:k1]^T
+struct GemmTile { int32_t front, r0, c0, cend, k0, k1; };
 struct DiagTask { int32_t front, j0, jb, slot; int32_t tile0, ntiles, pad0, pad1; };
 struct TrsmTile { int32_t front, j0, jb, r0, slot, local; };
 
 struct Range { int64_t begin = 0, count = 0; };
 
-struct StepPlan {
-  double upd_flops = 0;  // algorithmic flops of the update GEMM: 2 (m - j0) jb j0 per front
-  Range upd;    // left-looking update of the block column (GemmTile)
-  Range diag;   // diagonal block tasks (DiagTask)
-  Range trsm;   // row tiles below the block (TrsmTile)
+enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm };
+
+struct Phase {
+  PhaseKind kind;
+  Range r;
+  double flops = 0;  // algorithmic flops (GEMM phases)
+};
+
+struct SolveStep {
+  Range diag;   // DiagTask of one inner block for every front of the level that has it
+  Range rows;   // solve row tiles (TrsmTile, SR rows each) below those blocks
 };
 
 struct LevelPlan {
@@ -51,11 +65,9 @@ struct LevelPlan {
   Range zero;      // ZeroTile
   Range scatter;   // ScatterTile
   Range ea[2];     // EaTile per child slot
-  std::vector<StepPlan> steps;
-  Range syrk;      // GemmTile (lower tiles of the update matrices)
-  double syrk_flops = 0;  // algorithmic flops: u (u + 1) k per front (lower triangle)
+  std::vector<Phase> phases;       // factorization, in launch order
+  std::vector<SolveStep> solve;    // inner blocks in pivot order
   Range fronts_r;  // range in the level-ordered front list (solve kernels)
-  Range child_r[2];  // fronts of this level that are child slot s of their parent... (unused for now)
 };
 
 struct Plan {
@@ -71,14 +83,14 @@ struct Plan {
   std::vector<EaTile> ea;
   std::vector<GemmTile> gemm;
   std::vector<DiagTask> diag;
-  std::vector<TrsmTile> trsm;
+  std::vector<TrsmTile> trsm;   // TRS-row tiles of the factorization
+  std::vector<TrsmTile> srow;   // SR-row tiles of the solves
   std::vector<int32_t> child_slot_fronts[2];  // fronts that are child slot s (per level ranges below)
   std::vector<Range> child_slot_range[2];     // per level
   int64_t arena_elems = 0;
   int64_t tvec_elems = 0;  // sum of m
-  int32_t max_slots = 0;   // most diag tasks in one step
-  int32_t max_tiles_per_task = 0;
-  int64_t max_partials = 0;  // most trsm tiles in one step
+  int32_t max_slots = 0;   // most diag tasks in one inner step
+  int64_t max_partials = 0;  // most solve row tiles in one inner step
   double flops = 0.0;        // sum k^3/3 + u k^2 + u^2 k
 };
 

@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
 // ------------------------------------------------------------ pivot blocks
 
 // LDL^T of the jb x jb diagonal block in FP64 with the pivot test of
-// src/sparse.rs:528-531; the failing equation with the smallest index is kept.
+// src/sparse.rs:528-531 (the failing equation with the smallest index is
+// kept), then W = L^-T D^-1 of the block in FP64 for the TRSM (column-major
+// NB x NB in l11buf, zero outside the block).
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
                                               T* __restrict__ arena, double* __restrict__ dpiv,
@@ -130,6 +132,7 @@ __global__ __launch_bounds__(256) void k_diag(const FrontDesc* __restrict__ fd, 
   const FrontDesc d = fd[t.front];
   const int jb = t.jb;
   __shared__ double A[NB][NB + 1];  // A[col][row]
+  __shared__ double V[NB][NB + 1];  // V[col][row] = L^-1
   T* base = arena + d.off + t.j0 + static_cast<int64_t>(t.j0) * d.m;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < NB * NB; idx += 256) {
@@ -154,113 +157,164 @@ __global__ __launch_bounds__(256) void k_diag(const FrontDesc* __restrict__ fd, 
     }
     __syncthreads();
   }
+  // column q of L^-1 by forward substitution (one thread per column)
+  if (tid < NB) {
+    const int q = tid;
+    for (int i = 0; i < NB; ++i) V[q][i] = 0.0;
+    if (q < jb) {
+      V[q][q] = 1.0;
+      for (int i = q + 1; i < jb; ++i) {
+        double s = 0.0;
+        for (int p = q; p < i; ++p) s = fma(A[p][i], V[q][p], s);
+        V[q][i] = -s;
+      }
+    }
+  }
+  __syncthreads();
   double* lb = l11buf + static_cast<int64_t>(t.slot) * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int i = idx % NB, j = idx / NB;
-    const double v = (i > j) ? A[j][i] : 0.0;
-    lb[idx] = v;  // column-major, strictly lower, zero elsewhere
-    if (i < jb && j < jb && i > j) base[i + static_cast<int64_t>(j) * d.m] = T(v);
+    const int p = idx % NB, q = idx / NB;
+    // W[p][q] = L^-1[q][p] / d_q, stored column-major at p + q NB
+    lb[idx] = (p < jb && q < jb && p <= q) ? V[p][q] / A[q][q] : 0.0;
+    if (p < jb && q < jb && p > q) base[p + static_cast<int64_t>(q) * d.m] = T(A[q][p]);
   }
   if (tid < jb) dpiv[d.start + t.j0 + tid] = A[tid][tid];
 }
 
-// L[r, B] = F[r, B] L[B,B]^-T / D[B] for the rows of one tile, FP64 arithmetic
+// L[r, B] = F[r, B] W for the TRS rows of one tile (FP64 MFMA, in place)
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm(const FrontDesc* __restrict__ fd, const TrsmTile* __restrict__ tiles,
-                                              T* __restrict__ arena, const double* __restrict__ dpiv,
-                                              const double* __restrict__ l11buf) {
+                                              T* __restrict__ arena, const double* __restrict__ l11buf) {
+  using M = Mfma<double>;
   const TrsmTile t = tiles[blockIdx.x];
   const FrontDesc d = fd[t.front];
-  __shared__ double L[NB * NB];
-  __shared__ double dv[NB];
+  __shared__ double Ws[NB][NB + 2];   // Ws[p][q]
+  __shared__ double As[NB][TRS + 2];  // As[p][r]
   const double* lb = l11buf + static_cast<int64_t>(t.slot) * NB * NB;
-  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) L[idx] = lb[idx];
-  if (threadIdx.x < NB) dv[threadIdx.x] = threadIdx.x < t.jb ? dpiv[d.start + t.j0 + threadIdx.x] : 1.0;
-  __syncthreads();
-  const int r = t.r0 + threadIdx.x;
-  if (r >= d.m) return;
-  T* row = arena + d.off + r + static_cast<int64_t>(t.j0) * d.m;
-  const int jb = t.jb;
-  double w[NB];
-#pragma unroll
-  for (int p = 0; p < NB; ++p) w[p] = (p < jb) ? static_cast<double>(row[static_cast<int64_t>(p) * d.m]) : 0.0;
-#pragma unroll
-  for (int p = 0; p < NB; ++p) {
-    const double wp = -w[p];
-#pragma unroll
-    for (int q = p + 1; q < NB; ++q) w[q] = fma(wp, L[q + p * NB], w[q]);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  for (int idx = tid; idx < NB * NB; idx += 256) Ws[idx % NB][idx / NB] = lb[idx];
+  for (int idx = tid; idx < NB * TRS; idx += 256) {
+    const int r = idx % TRS, p = idx / TRS;
+    const int row = t.r0 + r;
+    As[p][r] = (row < d.m && p < t.jb) ? static_cast<double>(F[row + (t.j0 + p) * m]) : 0.0;
   }
+  __syncthreads();
+  M::acc_t acc[4];
 #pragma unroll
-  for (int p = 0; p < NB; ++p)
-    if (p < jb) row[static_cast<int64_t>(p) * d.m] = T(w[p] / dv[p]);
+  for (int bj = 0; bj < 4; ++bj) acc[bj] = M::acc_t{0, 0, 0, 0};
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    const double b = As[kk + lane / 16][wave * 16 + lane % 16];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bj] = M::op(Ws[kk + lane / 16][bj * 16 + lane % 16], b, acc[bj]);
+  }
+  const int row = t.r0 + wave * 16 + lane % 16;
+  if (row < d.m) {
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = bj * 16 + M::orow(lane, i);
+        if (q < t.jb) F[row + (t.j0 + q) * m] = T(acc[bj][i]);
+      }
+  }
 }
 
-// C[r0:, c0:cend) -= (L[r0:, 0:kdim] D) L[c0:cend, 0:kdim]^T on MFMA.
-// Tile 128 x 64, four waves of 32 x 64; the MFMA row index runs over C's
-// columns so that lanes 0..15 of an output register hold 16 consecutive rows
-// of the column-major front (coalesced read-modify-write).
-template <typename T>
+// C[r0:r0+128, c0:min(c0+TN, cend)) -= (L[r0:, k0:k1] D) L[c0:, k0:k1]^T on MFMA.
+// Four waves tile 128 x TN (TN = 64: 4 x 1 waves of 32 x 64; TN = 128: 2 x 2
+// waves of 64 x 64).  The MFMA row index runs over C's columns so that lanes
+// 0..15 of an output register hold 16 consecutive rows of the column-major
+// front (coalesced read-modify-write).  The next K tile is prefetched into
+// registers while the current one is multiplied from LDS.
+template <typename T, int TN>
 __global__ __launch_bounds__(256) void k_gemm(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
                                               T* __restrict__ arena, const double* __restrict__ dpiv) {
   using M = Mfma<T>;
   constexpr int KT = M::KT;
   constexpr int PAD = 4;
+  constexpr int WN = TN / 64;
+  constexpr int WM = 4 / WN;
+  constexpr int BI = GT_M / WM / 16;
+  constexpr int BJ = 4;
+  constexpr int NA = GT_M * KT / 256;
+  constexpr int NBL = TN * KT / 256;
   const GemmTile t = tiles[blockIdx.x];
   const FrontDesc d = fd[t.front];
   __shared__ T As[KT][GT_M + PAD];
-  __shared__ T Bs[KT][GT_N + PAD];
+  __shared__ T Bs[KT][TN + PAD];
+  __shared__ T ds[KT];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
   T* F = arena + d.off;
   const int64_t m = d.m;
-  typename M::acc_t acc[2][4];
+  typename M::acc_t acc[BI][BJ];
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = typename M::acc_t{0, 0, 0, 0};
-  for (int k0 = 0; k0 < t.kdim; k0 += KT) {
+    for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = typename M::acc_t{0, 0, 0, 0};
+  T ra[NA], rb[NBL], rd = T(0);
+  auto gload = [&](int kb) {
 #pragma unroll
-    for (int s = 0; s < GT_M * KT / 256; ++s) {
+    for (int s = 0; s < NA; ++s) {
       const int idx = tid + 256 * s;
       const int i = idx % GT_M, kk = idx / GT_M;
-      const int r = t.r0 + i, c = k0 + kk;
-      T v = T(0);
-      if (r < d.m && c < t.kdim) v = F[r + c * m] * static_cast<T>(dpiv[d.start + c]);
-      As[kk][i] = v;
+      const int r = t.r0 + i, c = kb + kk;
+      ra[s] = (r < d.m && c < t.k1) ? F[r + c * m] : T(0);
     }
 #pragma unroll
-    for (int s = 0; s < GT_N * KT / 256; ++s) {
+    for (int s = 0; s < NBL; ++s) {
       const int idx = tid + 256 * s;
-      const int j = idx % GT_N, kk = idx / GT_N;
-      const int r = t.c0 + j, c = k0 + kk;
-      T v = T(0);
-      if (r < t.cend && c < t.kdim) v = F[r + c * m];
-      Bs[kk][j] = v;
+      const int j = idx % TN, kk = idx / TN;
+      const int r = t.c0 + j, c = kb + kk;
+      rb[s] = (r < t.cend && c < t.k1) ? F[r + c * m] : T(0);
     }
+    if (tid < KT) rd = (kb + tid < t.k1) ? static_cast<T>(dpiv[d.start + kb + tid]) : T(0);
+  };
+  gload(t.k0);
+  for (int kb = t.k0; kb < t.k1; kb += KT) {
     __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      const int idx = tid + 256 * s;
+      As[idx / GT_M][idx % GT_M] = ra[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NBL; ++s) {
+      const int idx = tid + 256 * s;
+      Bs[idx / TN][idx % TN] = rb[s];
+    }
+    if (tid < KT) ds[tid] = rd;
+    __syncthreads();
+    if (kb + KT < t.k1) gload(kb + KT);
 #pragma unroll
     for (int kk = 0; kk < KT; kk += 4) {
-      T a[4], b[2];
+      const int kr = kk + lane / 16;
+      const T dk = ds[kr];
+      T a[BJ], b[BI];
 #pragma unroll
-      for (int bj = 0; bj < 4; ++bj) a[bj] = Bs[kk + lane / 16][bj * 16 + lane % 16];
+      for (int bj = 0; bj < BJ; ++bj) a[bj] = Bs[kr][wn * 64 + bj * 16 + lane % 16];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi) b[bi] = As[kk + lane / 16][wave * 32 + bi * 16 + lane % 16];
+      for (int bi = 0; bi < BI; ++bi) b[bi] = As[kr][wm * (GT_M / WM) + bi * 16 + lane % 16] * dk;
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+      for (int bi = 0; bi < BI; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(a[bj], b[bi], acc[bi][bj]);
+        for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = M::op(a[bj], b[bi], acc[bi][bj]);
     }
-    __syncthreads();
   }
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < BI; ++bi) {
+    const int row = t.r0 + wm * (GT_M / WM) + bi * 16 + lane % 16;
+    if (row >= d.m) continue;
 #pragma unroll
-    for (int bj = 0; bj < 4; ++bj)
+    for (int bj = 0; bj < BJ; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = t.r0 + wave * 32 + bi * 16 + lane % 16;
-        const int col = t.c0 + bj * 16 + M::orow(lane, r);
-        if (row < d.m && col < t.cend) F[row + col * m] -= acc[bi][bj][r];
+        const int col = t.c0 + wn * 64 + bj * 16 + M::orow(lane, r);
+        if (col < t.cend) F[row + col * m] -= acc[bi][bj][r];
       }
+  }
 }
 
 // ------------------------------------------------------------------ solves
@@ -575,6 +629,7 @@ class Impl final : public GpuFactor {
     d_gemm_ = dupload(P_.gemm, bytes_, s_);
     d_diag_ = dupload(P_.diag, bytes_, s_);
     d_trsm_ = dupload(P_.trsm, bytes_, s_);
+    d_srow_ = dupload(P_.srow, bytes_, s_);
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
     d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems), bytes_);
@@ -622,7 +677,7 @@ class Impl final : public GpuFactor {
 
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_srow_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_l11_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_};
     for (void* p : ptrs)
@@ -647,13 +702,21 @@ class Impl final : public GpuFactor {
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
-      for (const StepPlan& sp : lp.steps) {
-        gemm(sp.upd, sp.upd_flops);
-        launch(k_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_dpiv_, d_adiag_, d_l11_,
-               d_err_);
-        launch(k_trsm<T>, sp.trsm.count, 256, d_fd_, d_trsm_ + sp.trsm.begin, d_arena_, d_dpiv_, d_l11_);
+      for (const Phase& ph : lp.phases) {
+        switch (ph.kind) {
+          case PhaseKind::Gemm64:
+          case PhaseKind::Gemm128:
+            gemm(ph);
+            break;
+          case PhaseKind::Diag:
+            launch(k_diag<T>, ph.r.count, 256, d_fd_, d_diag_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_l11_,
+                   d_err_);
+            break;
+          case PhaseKind::Trsm:
+            launch(k_trsm<T>, ph.r.count, 256, d_fd_, d_trsm_ + ph.r.begin, d_arena_, d_l11_);
+            break;
+        }
       }
-      gemm(lp.syrk, lp.syrk_flops);
     }
     HIPCHK(hipEventRecord(e1_, s_));
     unsigned long long err = 0;
@@ -731,7 +794,9 @@ class Impl final : public GpuFactor {
   }
 
   // one launch of the Schur-update GEMM over a tile range
-  void gemm(const Range& r, double flops) {
+  void gemm(const Phase& ph) {
+    const Range& r = ph.r;
+    const double flops = ph.flops;
     if (r.count <= 0) return;
     if (ktime_) {
       if (nev_ >= static_cast<int64_t>(evs_.size()) / 2) {
@@ -743,7 +808,10 @@ class Impl final : public GpuFactor {
       }
       HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
     }
-    launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    if (ph.kind == PhaseKind::Gemm64)
+      launch(k_gemm<T, 64>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else
+      launch(k_gemm<T, 128>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
@@ -813,9 +881,9 @@ class Impl final : public GpuFactor {
         const Range& cr = P_.child_slot_range[sl][L];
         launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
       }
-      for (const StepPlan& sp : lp.steps) {
+      for (const SolveStep& sp : lp.solve) {
         launch(k_fwd_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_t_, tv, nrhs);
-        launch(k_fwd_update<T>, sp.trsm.count, 256, d_fd_, d_trsm_ + sp.trsm.begin, d_arena_, d_t_, tv, nrhs);
+        launch(k_fwd_update<T>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_t_, tv, nrhs);
       }
       launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
     }
@@ -823,9 +891,9 @@ class Impl final : public GpuFactor {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
       launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
-      for (int64_t s = static_cast<int64_t>(lp.steps.size()) - 1; s >= 0; --s) {
-        const StepPlan& sp = lp.steps[s];
-        launch(k_bwd_partial<T>, sp.trsm.count, 256, d_fd_, d_trsm_ + sp.trsm.begin, d_arena_, d_t_, tv, nrhs,
+      for (int64_t s = static_cast<int64_t>(lp.solve.size()) - 1; s >= 0; --s) {
+        const SolveStep& sp = lp.solve[s];
+        launch(k_bwd_partial<T>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_t_, tv, nrhs,
                d_part_);
         launch(k_bwd_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_t_, tv, nrhs,
                d_part_, static_cast<int64_t>(0));
@@ -854,6 +922,7 @@ class Impl final : public GpuFactor {
   GemmTile* d_gemm_ = nullptr;
   DiagTask* d_diag_ = nullptr;
   TrsmTile* d_trsm_ = nullptr;
+  TrsmTile* d_srow_ = nullptr;
   int32_t* d_lf_ = nullptr;
   int32_t* d_cs_[2] = {nullptr, nullptr};
   T* d_arena_ = nullptr;

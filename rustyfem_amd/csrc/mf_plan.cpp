@@ -108,54 +108,77 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
       lp.ea[s] = take(b, static_cast<int64_t>(P.ea.size()));
       P.child_slot_range[s][L] = take(cb, static_cast<int64_t>(P.child_slot_fronts[s].size()));
     }
-    // pivot-block steps
+    // pivot columns: outer blocks of NBO, inner blocks of NB
     int32_t maxk = 0;
     for (int32_t f : lp.fronts) maxk = std::max(maxk, P.fd[f].k);
-    const int32_t nsteps = (maxk + NB - 1) / NB;
-    lp.steps.resize(nsteps);
-    for (int32_t s = 0; s < nsteps; ++s) {
-      StepPlan& sp = lp.steps[s];
-      const int32_t j0 = s * NB;
-      b = static_cast<int64_t>(P.gemm.size());
-      if (j0 > 0)
+    auto phase = [&](PhaseKind kind, int64_t b0, int64_t b1, double flops) {
+      if (b1 > b0) lp.phases.push_back(Phase{kind, take(b0, b1), flops});
+    };
+    for (int32_t J0 = 0; J0 < maxk; J0 += NBO) {
+      for (int32_t j0 = J0; j0 < std::min(J0 + NBO, maxk); j0 += NB) {
+        // (a) left-looking update of the inner block from the outer block's earlier columns
+        int64_t b = static_cast<int64_t>(P.gemm.size());
+        double fl = 0;
+        if (j0 > J0)
+          for (int32_t f : lp.fronts) {
+            const FrontDesc& d = P.fd[f];
+            if (d.k <= j0) continue;
+            const int32_t j1 = std::min(j0 + NB, d.k);
+            for (int32_t r0 = j0; r0 < d.m; r0 += GT_M) P.gemm.push_back({f, r0, j0, j1, J0, j0});
+            fl += 2.0 * (d.m - j0) * static_cast<double>(j1 - j0) * (j0 - J0);
+          }
+        phase(PhaseKind::Gemm64, b, static_cast<int64_t>(P.gemm.size()), fl);
+        // (b) diagonal blocks, (c) TRSM row tiles; the solve's row tiles alongside
+        const int64_t db = static_cast<int64_t>(P.diag.size());
+        const int64_t tb = static_cast<int64_t>(P.trsm.size());
+        const int64_t sb = static_cast<int64_t>(P.srow.size());
+        int32_t slot = 0;
         for (int32_t f : lp.fronts) {
           const FrontDesc& d = P.fd[f];
           if (d.k <= j0) continue;
-          const int32_t j1 = std::min(j0 + NB, d.k);
-          for (int32_t r0 = j0; r0 < d.m; r0 += GT_M) P.gemm.push_back({f, r0, j0, j1, j0, 0});
-          sp.upd_flops += 2.0 * (d.m - j0) * static_cast<double>(j1 - j0) * j0;
+          const int32_t jb = std::min(NB, d.k - j0);
+          const int32_t j1 = j0 + jb;
+          for (int32_t r0 = j1; r0 < d.m; r0 += TRS) P.trsm.push_back({f, j0, jb, r0, slot, 0});
+          const int32_t t0 = static_cast<int32_t>(P.srow.size() - sb);
+          int32_t local = 0;
+          for (int32_t r0 = j1; r0 < d.m; r0 += SR) P.srow.push_back({f, j0, jb, r0, slot, local++});
+          P.diag.push_back({f, j0, jb, slot, t0, local, 0, 0});
+          ++slot;
         }
-      sp.upd = take(b, static_cast<int64_t>(P.gemm.size()));
-      const int64_t db = static_cast<int64_t>(P.diag.size());
-      const int64_t tb = static_cast<int64_t>(P.trsm.size());
-      int32_t slot = 0;
+        phase(PhaseKind::Diag, db, static_cast<int64_t>(P.diag.size()), 0);
+        phase(PhaseKind::Trsm, tb, static_cast<int64_t>(P.trsm.size()), 0);
+        lp.solve.push_back(SolveStep{take(db, static_cast<int64_t>(P.diag.size())),
+                                     take(sb, static_cast<int64_t>(P.srow.size()))});
+        P.max_slots = std::max(P.max_slots, slot);
+        P.max_partials = std::max<int64_t>(P.max_partials, static_cast<int64_t>(P.srow.size()) - sb);
+      }
+      // right-looking update of the remaining pivot columns [J1, k), K = [J0, J1)
+      const int32_t J1 = J0 + NBO;
+      int64_t b = static_cast<int64_t>(P.gemm.size());
+      double fl = 0;
       for (int32_t f : lp.fronts) {
         const FrontDesc& d = P.fd[f];
-        if (d.k <= j0) continue;
-        const int32_t jb = std::min(NB, d.k - j0);
-        const int32_t j1 = j0 + jb;
-        const int32_t t0 = static_cast<int32_t>(P.trsm.size() - tb);
-        int32_t local = 0;
-        for (int32_t r0 = j1; r0 < d.m; r0 += TR) P.trsm.push_back({f, j0, jb, r0, slot, local++});
-        P.diag.push_back({f, j0, jb, slot, t0, local, 0, 0});
-        P.max_tiles_per_task = std::max(P.max_tiles_per_task, local);
-        ++slot;
+        if (d.k <= J1) continue;
+        for (int32_t c0 = J1; c0 < d.k; c0 += 128)
+          for (int32_t r0 = J1; r0 < d.m; r0 += GT_M)
+            if (c0 <= r0 + GT_M - 1) P.gemm.push_back({f, r0, c0, d.k, J0, J1});
+        const double w = d.k - J1, below = d.m - d.k;
+        fl += 2.0 * NBO * (w * (w + 1.0) / 2.0 + below * w);
       }
-      sp.diag = take(db, static_cast<int64_t>(P.diag.size()));
-      sp.trsm = take(tb, static_cast<int64_t>(P.trsm.size()));
-      P.max_slots = std::max(P.max_slots, slot);
-      P.max_partials = std::max<int64_t>(P.max_partials, sp.trsm.count);
+      phase(PhaseKind::Gemm128, b, static_cast<int64_t>(P.gemm.size()), fl);
     }
-    // Schur update of the update matrices: lower tiles of rows/cols [k, m)
+    // Schur update of the update matrices: lower tiles of rows/cols [k, m), K = [0, k)
     b = static_cast<int64_t>(P.gemm.size());
+    double fl = 0;
     for (int32_t f : lp.fronts) {
       const FrontDesc& d = P.fd[f];
       if (d.u == 0 || d.k == 0) continue;
-      lp.syrk_flops += static_cast<double>(d.u) * (d.u + 1.0) * d.k;
-      for (int32_t r0 = d.k; r0 < d.m; r0 += GT_M)
-        for (int32_t c0 = d.k; c0 < d.m && c0 <= r0 + GT_M - 1; c0 += GT_N) P.gemm.push_back({f, r0, c0, d.m, d.k, 0});
+      fl += static_cast<double>(d.u) * (d.u + 1.0) * d.k;
+      for (int32_t c0 = d.k; c0 < d.m; c0 += 128)
+        for (int32_t r0 = d.k; r0 < d.m; r0 += GT_M)
+          if (c0 <= r0 + GT_M - 1) P.gemm.push_back({f, r0, c0, d.m, 0, d.k});
     }
-    lp.syrk = take(b, static_cast<int64_t>(P.gemm.size()));
+    phase(PhaseKind::Gemm128, b, static_cast<int64_t>(P.gemm.size()), fl);
   }
   return P;
 }
