@@ -24,7 +24,8 @@ namespace rf {
 constexpr int NB = 64;        // inner pivot block (diagonal factor, TRSM, solve)
 constexpr int NBO = 256;      // outer pivot block (right-looking updates)
 constexpr int TRS = 64;       // rows per TRSM tile
-constexpr int SR = 256;       // rows per solve tile
+constexpr int SR = 1024;      // rows per solve tile (4 per thread)
+constexpr int BC = 256;       // columns per backward tile (one per thread)
 constexpr int GT_M = 128;     // GEMM tile rows
 constexpr int EA_T = 64;      // extend-add tile edge
 constexpr int ZC = 64;        // columns per zero tile
@@ -42,12 +43,30 @@ struct ScatterTile { int64_t p0, p1; };
 struct EaTile { int32_t child, x0, y0, pad; };
 // C[r0:r0+128, c0:min(c0+TN, cend)) -= L[r0:, k0:k1] D[k0:k1] L[c0:, k0:k1]^T
 struct GemmTile { int32_t front, r0, c0, cend, k0, k1; };
-struct DiagTask { int32_t front, j0, jb, slot; int32_t tile0, ntiles, pad0, pad1; };
-struct TrsmTile { int32_t front, j0, jb, r0, slot, local; };
+// one NB-wide pivot block of one front; its index is the slot of the block's
+// FP64 inverse in the persistent V buffer (used by the TRSM and the solves)
+struct DiagTask { int32_t front, j0, jb, tile0, ntiles, pad; };
+struct TrsmTile { int32_t front, j0, jb, r0, task, pad; };
+// one inner step of one front for the rows [r0, r0 + TRS): left-looking update
+// from the outer block's columns [J0, j0), LDL^T of the diagonal block and
+// TRSM; the tile with r0 == j0 is the diagonal block itself and writes L_BB,
+// D and the block inverse (slot `task`)
+struct PanelTile { int32_t front, J0, j0, jb, r0, task; };
+// solve row tile below pivot block `task`; `next` is the front's following
+// block (-1 if none), `local`/`ntiles` its position among the block's tiles
+struct SolveTile { int32_t front, j0, jb, r0, task, next, local, ntiles; };
+// backward step: columns [c0, c1) of front `front` receive -L[B, c0:c1]^T z_B
+// for the solved block B = [j0, j0 + jb); when c1 == j0 the tile then solves
+// the previous block (task `prev`, first column c1 - jbp)
+struct BwdTile { int32_t front, j0, jb, c0, c1, prev, jbp, pad; };
+// rows [r0, r0 + SR) x columns [c0, c0 + BC) of a front's L21 (partial slot = tile index)
+struct L21Tile { int32_t front, r0, c0, pad; };
+// sum of the L21 partials of columns [c0, c0 + BC) of a front: tiles [t0, t0 + nt) of the level
+struct L21Comb { int32_t front, c0, t0, nt; };
 
 struct Range { int64_t begin = 0, count = 0; };
 
-enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm };
+enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel };
 
 struct Phase {
   PhaseKind kind;
@@ -57,7 +76,14 @@ struct Phase {
 
 struct SolveStep {
   Range diag;   // DiagTask of one inner block for every front of the level that has it
-  Range rows;   // solve row tiles (TrsmTile, SR rows each) below those blocks
+  Range rows;   // solve row tiles (SolveTile, SR rows each, at least one per block) below those blocks
+};
+
+struct BwdPlan {
+  Range l21;     // L21Tile partials of s = t[0:k] - L21^T t[k:m]
+  Range comb;    // L21Comb
+  Range last;    // DiagTask indices (into bwd_last) of each front's last block
+  std::vector<Range> steps;  // BwdTile ranges, step s handles block nb_f - 1 - s of every front
 };
 
 struct LevelPlan {
@@ -66,7 +92,8 @@ struct LevelPlan {
   Range scatter;   // ScatterTile
   Range ea[2];     // EaTile per child slot
   std::vector<Phase> phases;       // factorization, in launch order
-  std::vector<SolveStep> solve;    // inner blocks in pivot order
+  std::vector<SolveStep> solve;    // inner blocks in pivot order (forward)
+  BwdPlan bwd;                     // backward
   Range fronts_r;  // range in the level-ordered front list (solve kernels)
 };
 
@@ -84,13 +111,17 @@ struct Plan {
   std::vector<GemmTile> gemm;
   std::vector<DiagTask> diag;
   std::vector<TrsmTile> trsm;   // TRS-row tiles of the factorization
-  std::vector<TrsmTile> srow;   // SR-row tiles of the solves
+  std::vector<PanelTile> panel; // fused inner-step tiles of the factorization
+  std::vector<SolveTile> srow;  // SR-row tiles of the forward solve
+  std::vector<BwdTile> btile;   // column tiles of the backward solve
+  std::vector<L21Tile> l21;
+  std::vector<L21Comb> l21c;
+  std::vector<int32_t> bwd_last;  // DiagTask index of each front's last block, per level
   std::vector<int32_t> child_slot_fronts[2];  // fronts that are child slot s (per level ranges below)
   std::vector<Range> child_slot_range[2];     // per level
   int64_t arena_elems = 0;
   int64_t tvec_elems = 0;  // sum of m
-  int32_t max_slots = 0;   // most diag tasks in one inner step
-  int64_t max_partials = 0;  // most solve row tiles in one inner step
+  int64_t max_l21 = 0;       // most L21 partial tiles in one level
   double flops = 0.0;        // sum k^3/3 + u k^2 + u^2 k
 };
 

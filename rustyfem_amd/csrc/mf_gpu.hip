@@ -16,6 +16,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -121,80 +122,90 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
 
 // LDL^T of the jb x jb diagonal block in FP64 with the pivot test of
 // src/sparse.rs:528-531 (the failing equation with the smallest index is
-// kept), then W = L^-T D^-1 of the block in FP64 for the TRSM (column-major
-// NB x NB in l11buf, zero outside the block).
+// kept), then the block's inverse L^-1 in FP64 (column-major NB x NB in the
+// persistent V buffer at the task's index, zero outside the block).
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
-                                              T* __restrict__ arena, double* __restrict__ dpiv,
-                                              const double* __restrict__ adiag, double* __restrict__ l11buf,
+                                              int64_t task0, T* __restrict__ arena, double* __restrict__ dpiv,
+                                              const double* __restrict__ adiag, double* __restrict__ vbuf,
                                               unsigned long long* __restrict__ err) {
-  const DiagTask t = tasks[blockIdx.x];
+  const int64_t task = task0 + blockIdx.x;
+  const DiagTask t = tasks[task];
   const FrontDesc d = fd[t.front];
   const int jb = t.jb;
   __shared__ double A[NB][NB + 1];  // A[col][row]
   __shared__ double V[NB][NB + 1];  // V[col][row] = L^-1
+  __shared__ double ao[NB];         // original diagonal of the block's equations
   T* base = arena + d.off + t.j0 + static_cast<int64_t>(t.j0) * d.m;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < NB * NB; idx += 256) {
     const int i = idx % NB, j = idx / NB;
     A[j][i] = (i < jb && j < jb && i >= j) ? static_cast<double>(base[i + static_cast<int64_t>(j) * d.m]) : 0.0;
   }
+  if (tid < NB) ao[tid] = tid < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + tid] : 1.0;
   __syncthreads();
+  // right-looking elimination with the unscaled column j: A[q][i] -= A[j][i] A[j][q] / d_j
   for (int j = 0; j < jb; ++j) {
     const double dj = A[j][j];
-    if (tid == 0) {
-      const int64_t eq = static_cast<int64_t>(d.start) + t.j0 + j;
-      const double orig = adiag[eq];
-      if (!(orig > 0.0) || !(dj > 1e-12 * orig)) atomicMin(err, static_cast<unsigned long long>(eq));
-    }
-    __syncthreads();
-    for (int i = j + 1 + tid; i < jb; i += 256) A[j][i] = A[j][i] / dj;
-    __syncthreads();
+    if (tid == 0 && (!(ao[j] > 0.0) || !(dj > 1e-12 * ao[j])))
+      atomicMin(err, static_cast<unsigned long long>(static_cast<int64_t>(d.start) + t.j0 + j));
     const int rem = jb - j - 1;
     for (int idx = tid; idx < rem * rem; idx += 256) {
       const int i = j + 1 + idx % rem, q = j + 1 + idx / rem;
-      if (q <= i) A[q][i] -= A[j][i] * dj * A[j][q];
+      if (q <= i) A[q][i] -= A[j][i] * A[j][q] / dj;
     }
     __syncthreads();
   }
-  // column q of L^-1 by forward substitution (one thread per column)
-  if (tid < NB) {
-    const int q = tid;
-    for (int i = 0; i < NB; ++i) V[q][i] = 0.0;
-    if (q < jb) {
-      V[q][q] = 1.0;
-      for (int i = q + 1; i < jb; ++i) {
-        double s = 0.0;
-        for (int p = q; p < i; ++p) s = fma(A[p][i], V[q][p], s);
-        V[q][i] = -s;
-      }
+  // L = column-scaled A
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int i = idx % NB, j = idx / NB;
+    if (i > j && i < jb) A[j][i] = A[j][i] / A[j][j];
+  }
+  __syncthreads();
+  // L^-1 by forward substitution, four lanes per column (q = tid / 4), no block barriers
+  {
+    const int q = tid >> 2, g = tid & 3;
+    for (int i = g; i < NB; i += 4) V[q][i] = (i == q && q < jb) ? 1.0 : 0.0;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = 1; i < jb; ++i) {
+      double part = 0.0;
+      if (i > q)
+        for (int p = q + g; p < i; p += 4) part = fma(A[p][i], V[q][p], part);
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      if (g == 0 && i > q && q < jb) V[q][i] = -part;
+      __builtin_amdgcn_wave_barrier();
     }
   }
   __syncthreads();
-  double* lb = l11buf + static_cast<int64_t>(t.slot) * NB * NB;
+  double* vb = vbuf + task * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int p = idx % NB, q = idx / NB;
-    // W[p][q] = L^-1[q][p] / d_q, stored column-major at p + q NB
-    lb[idx] = (p < jb && q < jb && p <= q) ? V[p][q] / A[q][q] : 0.0;
-    if (p < jb && q < jb && p > q) base[p + static_cast<int64_t>(q) * d.m] = T(A[q][p]);
+    const int i = idx % NB, q = idx / NB;
+    vb[idx] = V[q][i];  // L^-1[i][q], column-major, zero outside the block
+    if (i < jb && q < jb && i > q) base[i + static_cast<int64_t>(q) * d.m] = T(A[q][i]);
   }
   if (tid < jb) dpiv[d.start + t.j0 + tid] = A[tid][tid];
 }
 
-// L[r, B] = F[r, B] W for the TRS rows of one tile (FP64 MFMA, in place)
+// L[r, B] = F[r, B] W with W = L_BB^-T D_B^-1, for the TRS rows of one tile
+// (FP64 MFMA, in place)
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm(const FrontDesc* __restrict__ fd, const TrsmTile* __restrict__ tiles,
-                                              T* __restrict__ arena, const double* __restrict__ l11buf) {
+                                              T* __restrict__ arena, const double* __restrict__ vbuf,
+                                              const double* __restrict__ dpiv) {
   using M = Mfma<double>;
   const TrsmTile t = tiles[blockIdx.x];
   const FrontDesc d = fd[t.front];
   __shared__ double Ws[NB][NB + 2];   // Ws[p][q]
   __shared__ double As[NB][TRS + 2];  // As[p][r]
-  const double* lb = l11buf + static_cast<int64_t>(t.slot) * NB * NB;
+  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   T* F = arena + d.off;
   const int64_t m = d.m;
-  for (int idx = tid; idx < NB * NB; idx += 256) Ws[idx % NB][idx / NB] = lb[idx];
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int q = idx % NB, p = idx / NB;  // vb[q + p NB] = L^-1[q][p]
+    Ws[p][q] = q < t.jb ? vb[idx] / dpiv[d.start + t.j0 + q] : 0.0;
+  }
   for (int idx = tid; idx < NB * TRS; idx += 256) {
     const int r = idx % TRS, p = idx / TRS;
     const int row = t.r0 + r;
@@ -218,6 +229,173 @@ __global__ __launch_bounds__(256) void k_trsm(const FrontDesc* __restrict__ fd, 
       for (int i = 0; i < 4; ++i) {
         const int q = bj * 16 + M::orow(lane, i);
         if (q < t.jb) F[row + (t.j0 + q) * m] = T(acc[bj][i]);
+      }
+  }
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// One inner pivot step of one front for one tile of rows (PanelTile):
+//  1. left-looking update from the outer block's columns [J0, j0) of the
+//     diagonal block (C_BB) and of the tile's rows (C_R), FP64 MFMA;
+//  2. LDL^T of the 64 x 64 diagonal block and its inverse L^-1, in the
+//     registers of wave 0 (lane i holds row i; rows are broadcast with
+//     readlane), with the pivot test of src/sparse.rs:528-531;
+//  3. the diagonal tile writes L_BB, D and L^-1; every other tile applies
+//     L[r, B] = A_R L_BB^-T D^-1 to its rows (FP64 MFMA) and writes them.
+// Every tile factors the (identical) diagonal block itself, so one launch
+// covers the whole inner step.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
+                                               T* __restrict__ arena, double* __restrict__ dpiv,
+                                               const double* __restrict__ adiag, double* __restrict__ vbuf,
+                                               unsigned long long* __restrict__ err) {
+  using M = Mfma<double>;
+  constexpr int KS = 32;
+  const PanelTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool diag_tile = t.r0 == t.j0;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  __shared__ double Ad[NB][NB + 1];   // diagonal block [col][row]; later L / D
+  __shared__ double Ar[NB][NB + 1];   // the tile's rows [col][row]; later W
+  __shared__ double Vs[NB][NB + 1];   // L^-1 [col][row]
+  __shared__ double Ys[KS][NB + 2];   // L[j0 + q, k] d_k
+  __shared__ double Y0[KS][NB + 2];   // L[j0 + i, k]
+  __shared__ double Xs[KS][NB + 2];   // L[r0 + i, k]
+  M::acc_t cb[4], cr[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) {
+    cb[bj] = M::acc_t{0, 0, 0, 0};
+    cr[bj] = M::acc_t{0, 0, 0, 0};
+  }
+  for (int k0 = t.J0; k0 < t.j0; k0 += KS) {
+    __syncthreads();
+    for (int e = tid; e < KS * NB; e += 256) {
+      const int i = e % NB, kk = e / NB;
+      const int k = k0 + kk;
+      const bool kin = k < t.j0;
+      const double y = (kin && i < jb) ? static_cast<double>(F[t.j0 + i + k * m]) : 0.0;
+      Y0[kk][i] = y;
+      Ys[kk][i] = kin ? y * dpiv[d.start + k] : 0.0;
+      Xs[kk][i] = (kin && !diag_tile && t.r0 + i < d.m) ? static_cast<double>(F[t.r0 + i + k * m]) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+      const int kr = kk + lane / 16;
+      const double b0 = Y0[kr][wave * 16 + lane % 16];
+      const double bx = Xs[kr][wave * 16 + lane % 16];
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        const double a = Ys[kr][bj * 16 + lane % 16];
+        cb[bj] = M::op(a, b0, cb[bj]);
+        cr[bj] = M::op(a, bx, cr[bj]);
+      }
+    }
+  }
+  // A_BB = F_BB - C_BB (both triangles from the stored lower triangle); A_R = F_R - C_R
+  __syncthreads();
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      const int i = wave * 16 + lane % 16;
+      double fb = 0.0;
+      if (i < jb && q < jb) {
+        const int hi = i > q ? i : q, lo = i > q ? q : i;
+        fb = static_cast<double>(F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m]) - cb[bj][r];
+      }
+      Ad[q][i] = fb;
+      if (!diag_tile) {
+        const int row = t.r0 + i;
+        Ar[q][i] = (row < d.m && q < jb) ? static_cast<double>(F[row + static_cast<int64_t>(t.j0 + q) * m]) - cr[bj][r]
+                                          : 0.0;
+      }
+    }
+  __syncthreads();
+  if (wave == 0) {
+    double a[NB], v[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) a[q] = Ad[q][lane];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {
+        const double dj = readlane_d(a[j], j);
+        if (diag_tile && lane == 0) {
+          const int64_t eq = static_cast<int64_t>(d.start) + t.j0 + j;
+          const double orig = adiag[eq];
+          if (!(orig > 0.0) || !(dj > 1e-12 * orig)) atomicMin(err, static_cast<unsigned long long>(eq));
+        }
+        const double l = a[j] / dj;
+#pragma unroll
+        for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, readlane_d(a[q], j), a[q]);
+        if (lane > j) a[j] = l;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) v[q] = q == lane ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      if (p < jb) {
+        const double lip = lane > p ? a[p] : 0.0;
+#pragma unroll
+        for (int q = 0; q <= p; ++q) v[q] = fma(-lip, readlane_d(v[q], p), v[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const bool in = lane < jb && q < jb;
+      Ad[q][lane] = in && q <= lane ? a[q] : 0.0;  // L strictly below, D on the diagonal
+      Vs[q][lane] = in && q <= lane ? v[q] : 0.0;  // L^-1[lane][q]
+    }
+  }
+  __syncthreads();
+  if (diag_tile) {
+    double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+      const int i = idx % NB, q = idx / NB;
+      vb[idx] = Vs[q][i];
+      if (i < jb && q < jb && i > q) F[t.j0 + i + static_cast<int64_t>(t.j0 + q) * m] = T(Ad[q][i]);
+    }
+    if (tid < jb) dpiv[d.start + t.j0 + tid] = Ad[tid][tid];
+    return;
+  }
+  // W[p][q] = L^-1[q][p] / d_q, kept in Ad's storage as Wd[p][q]
+  double (*Wd)[NB + 1] = Ad;
+  __shared__ double dinv[NB];
+  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / Ad[tid][tid] : 0.0;
+  __syncthreads();
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int q = idx % NB, p = idx / NB;
+    Wd[p][q] = q < jb ? Vs[p][q] : 0.0;  // Vs[p][q] = L^-1[q][p]
+  }
+  __syncthreads();
+  M::acc_t o[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    const int kr = kk + lane / 16;
+    const double b = Ar[kr][wave * 16 + lane % 16];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Wd[kr][bj * 16 + lane % 16], b, o[bj]);
+  }
+  const int row = t.r0 + wave * 16 + lane % 16;
+  if (row < d.m) {
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = bj * 16 + M::orow(lane, r);
+        if (q < jb) F[row + static_cast<int64_t>(t.j0 + q) * m] = T(o[bj][r] * dinv[q]);
       }
   }
 }
@@ -319,7 +497,8 @@ __global__ __launch_bounds__(256) void k_gemm(const FrontDesc* __restrict__ fd, 
 
 // ------------------------------------------------------------------ solves
 // Vectors t (per front, m entries, column per right-hand side) and x
-// (internal numbering, column per right-hand side) are FP64.
+// (internal numbering, column per right-hand side) are FP64.  Every pivot
+// block's triangular solve is a product with its FP64 inverse V = L_BB^-1.
 
 constexpr int RB = 16;  // right-hand sides per pass in the solve kernels
 
@@ -343,72 +522,99 @@ __global__ __launch_bounds__(256) void k_fwd_child(const FrontDesc* __restrict__
     for (int x = threadIdx.x; x < c.u; x += 256) t[j * tv + p.toff + cm[x]] += t[j * tv + c.toff + c.k + x];
 }
 
-// forward substitution with the unit lower diagonal block: y = L[B,B]^-1 t[B]
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_diag(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
-                                                  const T* __restrict__ arena, double* __restrict__ t, int64_t tv,
-                                                  int nrhs) {
-  const DiagTask task = tasks[blockIdx.x];
-  const FrontDesc d = fd[task.front];
-  __shared__ double L[NB][NB + 1];  // L[col][row]
-  __shared__ double y[NB][RB + 1];
-  const int jb = task.jb, tid = threadIdx.x;
-  const T* base = arena + d.off + task.j0 + static_cast<int64_t>(task.j0) * d.m;
-  for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int i = idx % NB, j = idx / NB;
-    L[j][i] = (i < jb && j < jb && i > j) ? static_cast<double>(base[i + static_cast<int64_t>(j) * d.m]) : 0.0;
-  }
-  for (int j0 = 0; j0 < nrhs; j0 += RB) {
-    const int nr = min(RB, nrhs - j0);
+// Vs <- V of one pivot block (global -> LDS)
+__device__ inline void load_v(const double* __restrict__ vb, double (*Vs)[NB + 1]) {
+  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) Vs[idx / NB][idx % NB] = vb[idx];  // Vs[q][i] = L^-1[i][q]
+}
+
+// t[B] <- V t[B] for one pivot block, R right-hand sides per pass
+template <int R>
+__device__ void block_lower_solve(const double (*Vs)[NB + 1], double* __restrict__ tb, int64_t tv, int jb, int nrhs,
+                                  double (*tmp)[R + 1]) {
+  const int tid = threadIdx.x;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
     __syncthreads();
-    for (int idx = tid; idx < NB * RB; idx += 256) {
+    for (int idx = tid; idx < NB * R; idx += 256) {
       const int i = idx % NB, j = idx / NB;
-      y[i][j] = (i < jb && j < nr) ? t[(j0 + j) * tv + d.toff + task.j0 + i] : 0.0;
+      tmp[i][j] = (i < jb && j < nr) ? tb[(j0 + j) * tv + i] : 0.0;
     }
     __syncthreads();
-    const int i = tid % NB, jq = tid / NB;
-    for (int p = 0; p < jb; ++p) {
-      if (i > p && i < jb)
-        for (int j = jq; j < nr; j += 4) y[i][j] -= L[p][i] * y[p][j];
-      __syncthreads();
-    }
-    for (int idx = tid; idx < NB * RB; idx += 256) {
-      const int ii = idx % NB, j = idx / NB;
-      if (ii < jb && j < nr) t[(j0 + j) * tv + d.toff + task.j0 + ii] = y[ii][j];
+    for (int idx = tid; idx < NB * R; idx += 256) {
+      const int i = idx % NB, j = idx / NB;
+      if (i < jb && j < nr) {
+        double s = 0.0;
+        for (int q = 0; q <= i; ++q) s = fma(Vs[q][i], tmp[q][j], s);
+        tb[(j0 + j) * tv + i] = s;
+      }
     }
   }
 }
 
-// t[r] -= L[r, B] y[B] for the rows of one tile
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_update(const FrontDesc* __restrict__ fd, const TrsmTile* __restrict__ tiles,
-                                                    const T* __restrict__ arena, double* __restrict__ t, int64_t tv,
-                                                    int nrhs) {
-  const TrsmTile tile = tiles[blockIdx.x];
+// first pivot block of every front of a level
+template <int R>
+__global__ __launch_bounds__(256) void k_fwd_first(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
+                                                   int64_t task0, const double* __restrict__ vbuf,
+                                                   double* __restrict__ t, int64_t tv, int nrhs) {
+  const int64_t task = task0 + blockIdx.x;
+  const DiagTask dt = tasks[task];
+  const FrontDesc d = fd[dt.front];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
+  load_v(vbuf + task * NB * NB, Vs);
+  block_lower_solve<R>(Vs, t + d.toff + dt.j0, tv, dt.jb, nrhs, tmp);
+}
+
+// t[r] -= L[r, B] y[B] for the SR rows of one tile (4 rows per thread); the
+// tile that starts at the next pivot block then solves that block (its rows
+// are final at this point).
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ fd, const SolveTile* __restrict__ tiles,
+                                                  const T* __restrict__ arena, const double* __restrict__ vbuf,
+                                                  double* __restrict__ t, int64_t tv, int nrhs) {
+  const SolveTile tile = tiles[blockIdx.x];
   const FrontDesc d = fd[tile.front];
-  __shared__ double y[RB][NB];
+  __shared__ double y[R][NB];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
   const int jb = tile.jb, tid = threadIdx.x;
-  const int r = tile.r0 + tid;
-  const T* row = arena + d.off + r + static_cast<int64_t>(tile.j0) * d.m;
-  for (int j0 = 0; j0 < nrhs; j0 += RB) {
-    const int nr = min(RB, nrhs - j0);
+  const bool solve_next = tile.local == 0 && tile.next >= 0;
+  if (solve_next) load_v(vbuf + static_cast<int64_t>(tile.next) * NB * NB, Vs);
+  const T* colb = arena + d.off + static_cast<int64_t>(tile.j0) * d.m;
+  double* tf = t + d.toff;
+  const int rend = min(tile.r0 + SR, d.m);
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
     __syncthreads();
-    for (int idx = tid; idx < NB * RB; idx += 256) {
+    for (int idx = tid; idx < NB * R; idx += 256) {
       const int p = idx % NB, j = idx / NB;
-      y[j][p] = (p < jb && j < nr) ? t[(j0 + j) * tv + d.toff + tile.j0 + p] : 0.0;
+      y[j][p] = (p < jb && j < nr) ? tf[(j0 + j) * tv + tile.j0 + p] : 0.0;
     }
     __syncthreads();
-    if (r < d.m) {
-      double acc[RB];
 #pragma unroll
-      for (int j = 0; j < RB; ++j) acc[j] = 0.0;
-      for (int p = 0; p < jb; ++p) {
-        const double l = static_cast<double>(row[static_cast<int64_t>(p) * d.m]);
+    for (int s4 = 0; s4 < SR / 256; ++s4) {
+      const int r = tile.r0 + tid + 256 * s4;
+      if (r < rend) {
+        double acc[R];
 #pragma unroll
-        for (int j = 0; j < RB; ++j) acc[j] = fma(l, y[j][p], acc[j]);
+        for (int j = 0; j < R; ++j) acc[j] = 0.0;
+        const T* row = colb + r;
+#pragma unroll 8
+        for (int p = 0; p < jb; ++p) {
+          const double l = static_cast<double>(row[static_cast<int64_t>(p) * d.m]);
+#pragma unroll
+          for (int j = 0; j < R; ++j) acc[j] = fma(l, y[j][p], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (j < nr) tf[(j0 + j) * tv + r] -= acc[j];
       }
-      for (int j = 0; j < nr; ++j) t[(j0 + j) * tv + d.toff + r] -= acc[j];
     }
+  }
+  if (solve_next) {
+    __syncthreads();
+    const int jbn = min(NB, d.k - (tile.j0 + jb));
+    block_lower_solve<R>(Vs, tf + tile.j0 + jb, tv, jbn, nrhs, tmp);
   }
 }
 
@@ -439,71 +645,169 @@ __global__ __launch_bounds__(256) void k_bwd_init(const FrontDesc* __restrict__ 
     }
 }
 
-// partial[tile][j][p] = sum over the tile's rows r of L[r, j0+p] t[r]
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_partial(const FrontDesc* __restrict__ fd, const TrsmTile* __restrict__ tiles,
-                                                     const T* __restrict__ arena, const double* __restrict__ t, int64_t tv,
-                                                     int nrhs, double* __restrict__ partial) {
-  const TrsmTile tile = tiles[blockIdx.x];
+// partial[tile][j][c] = sum over rows r of the tile of L21[r, c] t[r][j] for
+// the tile's BC columns; L is staged through LDS in 32-row slabs loaded with
+// lanes along the (contiguous) rows, then each thread sums one column.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_l21_partial(const FrontDesc* __restrict__ fd, const L21Tile* __restrict__ tiles,
+                                                     const T* __restrict__ arena, const double* __restrict__ t,
+                                                     int64_t tv, int nrhs, double* __restrict__ partial) {
+  const L21Tile tile = tiles[blockIdx.x];
   const FrontDesc d = fd[tile.front];
-  __shared__ double red[4][NB];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r = tile.r0 + tid;
-  const T* row = arena + d.off + r + static_cast<int64_t>(tile.j0) * d.m;
-  double* out = partial + static_cast<int64_t>(blockIdx.x) * nrhs * NB;
-  for (int j = 0; j < nrhs; ++j) {
-    const double tr = r < d.m ? t[j * tv + d.toff + r] : 0.0;
-    for (int p = 0; p < tile.jb; ++p) {
-      double v = r < d.m ? static_cast<double>(row[static_cast<int64_t>(p) * d.m]) * tr : 0.0;
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0) red[wave][p] = v;
+  constexpr int SL = 32;
+  __shared__ T Ls[BC][SL + 1];
+  __shared__ double ts[SL][R];
+  const int tid = threadIdx.x;
+  const int rend = min(tile.r0 + SR, d.m);
+  const int cend = min(tile.c0 + BC, d.k);
+  const T* F = arena + d.off;
+  const double* tf = t + d.toff;
+  double* out = partial + static_cast<int64_t>(blockIdx.x) * nrhs * BC;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    for (int rs = tile.r0; rs < rend; rs += SL) {
+      __syncthreads();
+#pragma unroll 4
+      for (int e = tid; e < SL * BC; e += 256) {
+        const int rr = e % SL, cc = e / SL;
+        const int r = rs + rr, c = tile.c0 + cc;
+        Ls[cc][rr] = (r < rend && c < cend) ? F[r + static_cast<int64_t>(c) * d.m] : T(0);
+      }
+      for (int e = tid; e < SL * R; e += 256) {
+        const int rr = e % SL, j = e / SL;
+        ts[rr][j] = (rs + rr < rend && j < nr) ? tf[(j0 + j) * tv + rs + rr] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int rr = 0; rr < SL; ++rr) {
+        const double l = static_cast<double>(Ls[tid][rr]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(l, ts[rr][j], acc[j]);
+      }
     }
-    __syncthreads();
-    if (tid < tile.jb) out[j * NB + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j < nr) out[(j0 + j) * BC + tid] = acc[j];
   }
 }
 
-// t[B] = L[B,B]^-T (t[B] - sum of the partials), back substitution
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_diag(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
-                                                  const T* __restrict__ arena, double* __restrict__ t, int64_t tv,
-                                                  int nrhs, const double* __restrict__ partial, int64_t tile_base) {
-  const DiagTask task = tasks[blockIdx.x];
-  const FrontDesc d = fd[task.front];
-  __shared__ double L[NB][NB + 1];  // L[col][row]
-  __shared__ double z[NB][RB + 1];
-  const int jb = task.jb, tid = threadIdx.x;
-  const T* base = arena + d.off + task.j0 + static_cast<int64_t>(task.j0) * d.m;
-  for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int i = idx % NB, j = idx / NB;
-    L[j][i] = (i < jb && j < jb && i > j) ? static_cast<double>(base[i + static_cast<int64_t>(j) * d.m]) : 0.0;
+// t[c] -= sum of the L21 partials of column c (tile order) for BC columns of a front
+__global__ __launch_bounds__(256) void k_l21_combine(const FrontDesc* __restrict__ fd, const L21Comb* __restrict__ comb,
+                                                     const double* __restrict__ partial,
+                                                     double* __restrict__ t, int64_t tv, int nrhs) {
+  const L21Comb cb = comb[blockIdx.x];
+  const FrontDesc d = fd[cb.front];
+  const int c = cb.c0 + threadIdx.x;
+  if (c >= d.k) return;
+  for (int j = 0; j < nrhs; ++j) {
+    const double* src = partial + (cb.t0 * static_cast<int64_t>(nrhs) + j) * BC + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(nrhs) * BC;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int q = 0;
+    for (; q + 4 <= cb.nt; q += 4) {
+      a0 += src[(q + 0) * stride];
+      a1 += src[(q + 1) * stride];
+      a2 += src[(q + 2) * stride];
+      a3 += src[(q + 3) * stride];
+    }
+    for (; q < cb.nt; ++q) a0 += src[q * stride];
+    t[j * tv + d.toff + c] -= (a0 + a1) + (a2 + a3);
   }
-  for (int j0 = 0; j0 < nrhs; j0 += RB) {
-    const int nr = min(RB, nrhs - j0);
+}
+
+// t[B] <- V^T t[B] for one pivot block, R right-hand sides per pass
+template <int R>
+__device__ void block_upper_solve(const double (*Vs)[NB + 1], double* __restrict__ tb, int64_t tv, int jb, int nrhs,
+                                  double (*tmp)[R + 1]) {
+  const int tid = threadIdx.x;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
     __syncthreads();
-    for (int idx = tid; idx < NB * RB; idx += 256) {
+    for (int idx = tid; idx < NB * R; idx += 256) {
       const int i = idx % NB, j = idx / NB;
-      double v = 0.0;
-      if (i < jb && j < nr) {
-        v = t[(j0 + j) * tv + d.toff + task.j0 + i];
-        const double* pp = partial + (tile_base + task.tile0) * static_cast<int64_t>(nrhs) * NB;
-        for (int q = 0; q < task.ntiles; ++q) v -= pp[(static_cast<int64_t>(q) * nrhs + j0 + j) * NB + i];
-      }
-      z[i][j] = v;
+      tmp[i][j] = (i < jb && j < nr) ? tb[(j0 + j) * tv + i] : 0.0;
     }
     __syncthreads();
-    const int i = tid % NB, jq = tid / NB;
-    for (int p = jb - 1; p >= 0; --p) {
-      // z[p] is final; remove its contribution from rows i < p: z[i] -= L[p][i] z[p]
-      if (i < p)
-        for (int j = jq; j < nr; j += 4) z[i][j] -= L[i][p] * z[p][j];
+    for (int idx = tid; idx < NB * R; idx += 256) {
+      const int p = idx % NB, j = idx / NB;
+      if (p < jb && j < nr) {
+        double s = 0.0;
+        for (int i = p; i < jb; ++i) s = fma(Vs[p][i], tmp[i][j], s);
+        tb[(j0 + j) * tv + p] = s;
+      }
+    }
+  }
+}
+
+// the last pivot block of every front of a level: t[B] <- V^T t[B]
+template <int R>
+__global__ __launch_bounds__(256) void k_bwd_last(const FrontDesc* __restrict__ fd, const DiagTask* __restrict__ tasks,
+                                                  const int32_t* __restrict__ last, const double* __restrict__ vbuf,
+                                                  double* __restrict__ t, int64_t tv, int nrhs) {
+  const int64_t task = last[blockIdx.x];
+  const DiagTask dt = tasks[task];
+  const FrontDesc d = fd[dt.front];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
+  load_v(vbuf + task * NB * NB, Vs);
+  block_upper_solve<R>(Vs, t + d.toff + dt.j0, tv, dt.jb, nrhs, tmp);
+}
+
+// t[c] -= sum_i L[j0 + i, c] z_i for the tile's columns (z = t[B], solved); the
+// tile ending at the block then solves the previous block.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ fd, const BwdTile* __restrict__ tiles,
+                                                  const T* __restrict__ arena, const double* __restrict__ vbuf,
+                                                  double* __restrict__ t, int64_t tv, int nrhs) {
+  const BwdTile tile = tiles[blockIdx.x];
+  const FrontDesc d = fd[tile.front];
+  constexpr int SL = 32;
+  __shared__ T Ls[BC][SL + 1];
+  __shared__ double zs[NB][R];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
+  const int tid = threadIdx.x;
+  const T* F = arena + d.off;
+  double* tf = t + d.toff;
+  const int c = tile.c0 + tid;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int e = tid; e < NB * R; e += 256) {
+      const int i = e % NB, j = e / NB;
+      zs[i][j] = (i < tile.jb && j < nr) ? tf[(j0 + j) * tv + tile.j0 + i] : 0.0;
+    }
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    for (int is = 0; is < tile.jb; is += SL) {
       __syncthreads();
+#pragma unroll 4
+      for (int e = tid; e < SL * BC; e += 256) {
+        const int rr = e % SL, cc = e / SL;
+        const int i = is + rr, cg = tile.c0 + cc;
+        Ls[cc][rr] = (i < tile.jb && cg < tile.c1) ? F[tile.j0 + i + static_cast<int64_t>(cg) * d.m] : T(0);
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int rr = 0; rr < SL; ++rr) {
+        const double l = static_cast<double>(Ls[tid][rr]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(l, zs[is + rr][j], acc[j]);
+      }
     }
-    for (int idx = tid; idx < NB * RB; idx += 256) {
-      const int ii = idx % NB, j = idx / NB;
-      if (ii < jb && j < nr) t[(j0 + j) * tv + d.toff + task.j0 + ii] = z[ii][j];
-    }
+    if (c < tile.c1)
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j < nr) tf[(j0 + j) * tv + c] -= acc[j];
+  }
+  if (tile.prev >= 0) {
+    __syncthreads();
+    load_v(vbuf + static_cast<int64_t>(tile.prev) * NB * NB, Vs);
+    block_upper_solve<R>(Vs, tf + tile.c1 - tile.jbp, tv, tile.jbp, nrhs, tmp);
   }
 }
 
@@ -616,6 +920,8 @@ class Impl final : public GpuFactor {
  public:
   Impl(Plan plan, const std::vector<int64_t>& colptr, const std::vector<int64_t>& rowidx) : P_(std::move(plan)) {
     HIPCHK(hipStreamCreate(&s_));
+    const char* ng = std::getenv("RUSTYFEM_NO_GRAPH");
+    use_graphs_ = !(ng && *ng && *ng != '0');
     HIPCHK(hipEventCreate(&e0_));
     HIPCHK(hipEventCreate(&e1_));
     n_ = P_.n;
@@ -629,14 +935,20 @@ class Impl final : public GpuFactor {
     d_gemm_ = dupload(P_.gemm, bytes_, s_);
     d_diag_ = dupload(P_.diag, bytes_, s_);
     d_trsm_ = dupload(P_.trsm, bytes_, s_);
+    d_panel_ = dupload(P_.panel, bytes_, s_);
     d_srow_ = dupload(P_.srow, bytes_, s_);
+    d_btile_ = dupload(P_.btile, bytes_, s_);
+    d_l21_ = dupload(P_.l21, bytes_, s_);
+    d_l21c_ = dupload(P_.l21c, bytes_, s_);
+    d_last_ = dupload(P_.bwd_last, bytes_, s_);
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
     d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems), bytes_);
     d_dpiv_ = dalloc<double>(n_, bytes_);
     d_adiag_ = dalloc<double>(n_, bytes_);
     d_lval_ = dalloc<double>(nlower_, bytes_);
-    d_l11_ = dalloc<double>(static_cast<size_t>(std::max(P_.max_slots, 1)) * NB * NB, bytes_);
+    d_vbuf_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.diag.size(), 1)) * NB * NB, bytes_);
+
     d_err_ = dalloc<unsigned long long>(1, bytes_);
     // diagonal positions of the Lower pattern (first entry of every column)
     std::vector<int64_t> dpos(n_);
@@ -677,8 +989,9 @@ class Impl final : public GpuFactor {
 
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_srow_, d_lf_, d_cs_[0],
-                    d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_l11_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
+    drop_graphs();
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_lf_, d_cs_[0],
+                    d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
@@ -709,11 +1022,15 @@ class Impl final : public GpuFactor {
             gemm(ph);
             break;
           case PhaseKind::Diag:
-            launch(k_diag<T>, ph.r.count, 256, d_fd_, d_diag_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_l11_,
+            launch(k_diag<T>, ph.r.count, 256, d_fd_, d_diag_, ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
                    d_err_);
             break;
           case PhaseKind::Trsm:
-            launch(k_trsm<T>, ph.r.count, 256, d_fd_, d_trsm_ + ph.r.begin, d_arena_, d_l11_);
+            launch(k_trsm<T>, ph.r.count, 256, d_fd_, d_trsm_ + ph.r.begin, d_arena_, d_vbuf_, d_dpiv_);
+            break;
+          case PhaseKind::Panel:
+            launch(k_panel<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
+                   d_err_);
             break;
         }
       }
@@ -821,6 +1138,7 @@ class Impl final : public GpuFactor {
 
   void ensure_vectors(int nrhs) {
     if (nrhs <= cap_rhs_) return;
+    drop_graphs();
     for (void* p : {static_cast<void*>(d_t_), static_cast<void*>(d_x_), static_cast<void*>(d_part_),
                     static_cast<void*>(d_b_), static_cast<void*>(d_r_), static_cast<void*>(d_red_),
                     static_cast<void*>(d_nrm_), static_cast<void*>(d_bc_)})
@@ -830,7 +1148,7 @@ class Impl final : public GpuFactor {
     d_x_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_b_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_r_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
-    d_part_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.max_partials, 1)) * nrhs * NB, bytes_);
+    d_part_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.max_l21, 1)) * nrhs * BC, bytes_);
     d_red_ = dalloc<double>(static_cast<size_t>(kRedBlocks) * nrhs, bytes_);
     d_nrm_ = dalloc<double>(nrhs, bytes_);
     d_bc_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(nc_, 1)) * nrhs, bytes_);
@@ -871,7 +1189,52 @@ class Impl final : public GpuFactor {
     HIPCHK(hipStreamSynchronize(s_));
   }
 
+  void solve_eager(double* dx, int nrhs) {
+    if (nrhs == 1)
+      solve_levels<1>(dx, nrhs);
+    else if (nrhs <= 4)
+      solve_levels<4>(dx, nrhs);
+    else
+      solve_levels<16>(dx, nrhs);
+  }
+
+  // The solve is a fixed sequence of kernel launches on fixed buffers: it is
+  // captured once per (vector, nrhs) into a HIP graph and replayed.
   void solve_device(double* dx, int nrhs) {
+    if (!use_graphs_) {
+      solve_eager(dx, nrhs);
+      return;
+    }
+    for (const GraphEntry& e : graphs_)
+      if (e.x == dx && e.nrhs == nrhs) {
+        HIPCHK(hipGraphLaunch(e.exec, s_));
+        return;
+      }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+      solve_eager(dx, nrhs);
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(s_, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(s_, &g));
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    graphs_.push_back(GraphEntry{dx, nrhs, ge});
+    HIPCHK(hipGraphLaunch(ge, s_));
+  }
+
+  void drop_graphs() {
+    for (const GraphEntry& e : graphs_) (void)hipGraphExecDestroy(e.exec);
+    graphs_.clear();
+  }
+
+  template <int R>
+  void solve_levels(double* dx, int nrhs) {
     const int64_t tv = P_.tvec_elems;
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
@@ -881,23 +1244,24 @@ class Impl final : public GpuFactor {
         const Range& cr = P_.child_slot_range[sl][L];
         launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
       }
-      for (const SolveStep& sp : lp.solve) {
-        launch(k_fwd_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_t_, tv, nrhs);
-        launch(k_fwd_update<T>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_t_, tv, nrhs);
-      }
+      if (!lp.solve.empty())
+        launch(k_fwd_first<R>, lp.solve[0].diag.count, 256, d_fd_, d_diag_, lp.solve[0].diag.begin, d_vbuf_, d_t_,
+               tv, nrhs);
+      for (const SolveStep& sp : lp.solve)
+        launch(k_fwd_step<T, R>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_vbuf_, d_t_, tv,
+               nrhs);
       launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
+      const BwdPlan& bp = lp.bwd;
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
       launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
-      for (int64_t s = static_cast<int64_t>(lp.solve.size()) - 1; s >= 0; --s) {
-        const SolveStep& sp = lp.solve[s];
-        launch(k_bwd_partial<T>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_t_, tv, nrhs,
-               d_part_);
-        launch(k_bwd_diag<T>, sp.diag.count, 256, d_fd_, d_diag_ + sp.diag.begin, d_arena_, d_t_, tv, nrhs,
-               d_part_, static_cast<int64_t>(0));
-      }
+      launch(k_l21_partial<T, R>, bp.l21.count, 256, d_fd_, d_l21_ + bp.l21.begin, d_arena_, d_t_, tv, nrhs, d_part_);
+      launch(k_l21_combine, bp.comb.count, 256, d_fd_, d_l21c_ + bp.comb.begin, d_part_, d_t_, tv, nrhs);
+      launch(k_bwd_last<R>, bp.last.count, 256, d_fd_, d_diag_, d_last_ + bp.last.begin, d_vbuf_, d_t_, tv, nrhs);
+      for (const Range& st : bp.steps)
+        launch(k_bwd_step<T, R>, st.count, 256, d_fd_, d_btile_ + st.begin, d_arena_, d_vbuf_, d_t_, tv, nrhs);
       launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
     }
   }
@@ -909,6 +1273,13 @@ class Impl final : public GpuFactor {
   int64_t n_ = 0, nlower_ = 0, nfull_ = 0;
   size_t bytes_ = 0;
   Timings tm_;
+  struct GraphEntry {
+    double* x;
+    int nrhs;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs_;
+  bool use_graphs_ = true;
   bool ktime_ = false;
   std::vector<hipEvent_t> evs_;
   int64_t nev_ = 0;
@@ -922,11 +1293,17 @@ class Impl final : public GpuFactor {
   GemmTile* d_gemm_ = nullptr;
   DiagTask* d_diag_ = nullptr;
   TrsmTile* d_trsm_ = nullptr;
-  TrsmTile* d_srow_ = nullptr;
+  PanelTile* d_panel_ = nullptr;
+  SolveTile* d_srow_ = nullptr;
+  BwdTile* d_btile_ = nullptr;
+  L21Tile* d_l21_ = nullptr;
+  L21Comb* d_l21c_ = nullptr;
+  int32_t* d_last_ = nullptr;
   int32_t* d_lf_ = nullptr;
   int32_t* d_cs_[2] = {nullptr, nullptr};
   T* d_arena_ = nullptr;
-  double *d_dpiv_ = nullptr, *d_adiag_ = nullptr, *d_lval_ = nullptr, *d_l11_ = nullptr;
+  double *d_dpiv_ = nullptr, *d_adiag_ = nullptr, *d_lval_ = nullptr, *d_vbuf_ = nullptr;
+
   unsigned long long* d_err_ = nullptr;
   int64_t* d_dpos_ = nullptr;
   int64_t* d_rp_ = nullptr;

@@ -109,6 +109,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
       P.child_slot_range[s][L] = take(cb, static_cast<int64_t>(P.child_slot_fronts[s].size()));
     }
     // pivot columns: outer blocks of NBO, inner blocks of NB
+    std::vector<std::pair<int64_t, int32_t>> last_tiles(P.nf, {-1, 0});  // solve tiles of a front's latest block
     int32_t maxk = 0;
     for (int32_t f : lp.fronts) maxk = std::max(maxk, P.fd[f].k);
     auto phase = [&](PhaseKind kind, int64_t b0, int64_t b1, double flops) {
@@ -116,41 +117,33 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     };
     for (int32_t J0 = 0; J0 < maxk; J0 += NBO) {
       for (int32_t j0 = J0; j0 < std::min(J0 + NBO, maxk); j0 += NB) {
-        // (a) left-looking update of the inner block from the outer block's earlier columns
-        int64_t b = static_cast<int64_t>(P.gemm.size());
-        double fl = 0;
-        if (j0 > J0)
-          for (int32_t f : lp.fronts) {
-            const FrontDesc& d = P.fd[f];
-            if (d.k <= j0) continue;
-            const int32_t j1 = std::min(j0 + NB, d.k);
-            for (int32_t r0 = j0; r0 < d.m; r0 += GT_M) P.gemm.push_back({f, r0, j0, j1, J0, j0});
-            fl += 2.0 * (d.m - j0) * static_cast<double>(j1 - j0) * (j0 - J0);
-          }
-        phase(PhaseKind::Gemm64, b, static_cast<int64_t>(P.gemm.size()), fl);
-        // (b) diagonal blocks, (c) TRSM row tiles; the solve's row tiles alongside
         const int64_t db = static_cast<int64_t>(P.diag.size());
-        const int64_t tb = static_cast<int64_t>(P.trsm.size());
+        const int64_t pb = static_cast<int64_t>(P.panel.size());
         const int64_t sb = static_cast<int64_t>(P.srow.size());
-        int32_t slot = 0;
         for (int32_t f : lp.fronts) {
           const FrontDesc& d = P.fd[f];
           if (d.k <= j0) continue;
           const int32_t jb = std::min(NB, d.k - j0);
           const int32_t j1 = j0 + jb;
-          for (int32_t r0 = j1; r0 < d.m; r0 += TRS) P.trsm.push_back({f, j0, jb, r0, slot, 0});
+          const int32_t task = static_cast<int32_t>(P.diag.size());
+          P.panel.push_back({f, J0, j0, jb, j0, task});
+          for (int32_t r0 = j1; r0 < d.m; r0 += TRS) P.panel.push_back({f, J0, j0, jb, r0, task});
           const int32_t t0 = static_cast<int32_t>(P.srow.size() - sb);
           int32_t local = 0;
-          for (int32_t r0 = j1; r0 < d.m; r0 += SR) P.srow.push_back({f, j0, jb, r0, slot, local++});
-          P.diag.push_back({f, j0, jb, slot, t0, local, 0, 0});
-          ++slot;
+          // at least one (possibly empty) tile, so that the backward step of the block runs
+          for (int32_t r0 = j1; r0 < d.m || local == 0; r0 += SR) {
+            P.srow.push_back({f, j0, jb, std::min(r0, d.m), task, -1, local++, 0});
+            if (r0 >= d.m) break;
+          }
+          for (int32_t q = 0; q < local; ++q) P.srow[sb + t0 + q].ntiles = local;
+          if (last_tiles[f].first >= 0)  // the previous block's tiles compute this block's solve
+            for (int32_t q = 0; q < last_tiles[f].second; ++q) P.srow[last_tiles[f].first + q].next = task;
+          P.diag.push_back({f, j0, jb, t0, local, 0});
+          last_tiles[f] = {sb + t0, local};
         }
-        phase(PhaseKind::Diag, db, static_cast<int64_t>(P.diag.size()), 0);
-        phase(PhaseKind::Trsm, tb, static_cast<int64_t>(P.trsm.size()), 0);
+        phase(PhaseKind::Panel, pb, static_cast<int64_t>(P.panel.size()), 0);
         lp.solve.push_back(SolveStep{take(db, static_cast<int64_t>(P.diag.size())),
                                      take(sb, static_cast<int64_t>(P.srow.size()))});
-        P.max_slots = std::max(P.max_slots, slot);
-        P.max_partials = std::max<int64_t>(P.max_partials, static_cast<int64_t>(P.srow.size()) - sb);
       }
       // right-looking update of the remaining pivot columns [J1, k), K = [J0, J1)
       const int32_t J1 = J0 + NBO;
@@ -166,6 +159,56 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
         fl += 2.0 * NBO * (w * (w + 1.0) / 2.0 + below * w);
       }
       phase(PhaseKind::Gemm128, b, static_cast<int64_t>(P.gemm.size()), fl);
+    }
+    // backward solve of the level
+    {
+      BwdPlan& bp = lp.bwd;
+      const int64_t lb = static_cast<int64_t>(P.l21.size());
+      const int64_t cb = static_cast<int64_t>(P.l21c.size());
+      for (int32_t f : lp.fronts) {
+        const FrontDesc& d = P.fd[f];
+        if (d.k == 0 || d.u == 0) continue;
+        for (int32_t c0 = 0; c0 < d.k; c0 += BC) {
+          const int32_t t0 = static_cast<int32_t>(P.l21.size() - lb);
+          int32_t nt = 0;
+          for (int32_t r0 = d.k; r0 < d.m; r0 += SR, ++nt) P.l21.push_back({f, r0, c0, 0});
+          P.l21c.push_back({f, c0, t0, nt});
+        }
+      }
+      bp.l21 = take(lb, static_cast<int64_t>(P.l21.size()));
+      bp.comb = take(cb, static_cast<int64_t>(P.l21c.size()));
+      P.max_l21 = std::max(P.max_l21, bp.l21.count);
+      // every front's blocks in reverse, aligned at the last block
+      std::vector<std::vector<int32_t>> tasks(lp.fronts.size());
+      for (int64_t q = lp.solve.empty() ? 0 : lp.solve.front().diag.begin;
+           !lp.solve.empty() && q < lp.solve.back().diag.begin + lp.solve.back().diag.count; ++q) {
+        const int32_t f = P.diag[q].front;
+        const size_t li = std::lower_bound(lp.fronts.begin(), lp.fronts.end(), f) - lp.fronts.begin();
+        tasks[li].push_back(static_cast<int32_t>(q));
+      }
+      const int64_t kb = static_cast<int64_t>(P.bwd_last.size());
+      size_t nsteps = 0;
+      for (size_t li = 0; li < tasks.size(); ++li)
+        if (!tasks[li].empty()) {
+          P.bwd_last.push_back(tasks[li].back());
+          nsteps = std::max(nsteps, tasks[li].size());
+        }
+      bp.last = take(kb, static_cast<int64_t>(P.bwd_last.size()));
+      for (size_t st = 0; st + 1 < nsteps; ++st) {
+        const int64_t tb0 = static_cast<int64_t>(P.btile.size());
+        for (size_t li = 0; li < tasks.size(); ++li) {
+          const auto& tl = tasks[li];
+          if (tl.size() < st + 2) continue;
+          const DiagTask& B = P.diag[tl[tl.size() - 1 - st]];
+          const DiagTask& Bp = P.diag[tl[tl.size() - 2 - st]];
+          for (int32_t c1 = B.j0; c1 > 0; c1 -= BC) {
+            const int32_t c0 = std::max(0, c1 - BC);
+            const bool tail = c1 == B.j0;
+            P.btile.push_back({B.front, B.j0, B.jb, c0, c1, tail ? tl[tl.size() - 2 - st] : -1, tail ? Bp.jb : 0, 0});
+          }
+        }
+        bp.steps.push_back(take(tb0, static_cast<int64_t>(P.btile.size())));
+      }
     }
     // Schur update of the update matrices: lower tiles of rows/cols [k, m), K = [0, k)
     b = static_cast<int64_t>(P.gemm.size());

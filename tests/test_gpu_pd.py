@@ -114,3 +114,34 @@ def test_refactor_same_pattern(rf):
     assert h.refactor(None) == -1
     x3 = h.solve(p.rhs)
     assert np.allclose(x3, x2, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 17])
+def test_many_rhs_match_oracle(rf, ora, nrhs):
+    """Multi-RHS solves (passes of 1, 4 and 16 columns) against the oracle."""
+    p = meshes.hex_block(7, nrhs=nrhs, seed=5)
+    print(f"case nrhs={nrhs} n={p.n}", flush=True)
+    hg, _, _ = rf.Handle.factor(p)
+    ho, _, _ = ora.Handle.factor(p)
+    xg = hg.solve(p.rhs)
+    xo = ho.solve(p.rhs)
+    assert np.max(np.abs(xg - xo)) / np.max(np.abs(xo)) < 1e-10
+    for j in range(nrhs):
+        assert np.allclose(hg.solve(p.rhs[:, j]), xg[:, j], rtol=0, atol=1e-12 * np.max(np.abs(xg[:, j])))
+
+
+def test_graph_replays_match_eager(rf, monkeypatch):
+    """The captured solve graph, replayed several times with eager work and
+    synchronizations in between, gives exactly the eager solve's result."""
+    p = meshes.hex_block(6, nrhs=2)
+    monkeypatch.setenv("RUSTYFEM_NO_GRAPH", "1")
+    he, _, _ = rf.Handle.factor(p, precision="mixed")
+    monkeypatch.delenv("RUSTYFEM_NO_GRAPH")
+    hg, _, _ = rf.Handle.factor(p, precision="mixed")
+    ref = he.solve(p.rhs)
+    for rep in range(4):
+        x = hg.solve(p.rhs)
+        assert np.array_equal(x, ref), f"replay {rep}"
+        assert he.refactor(None) == -1 and hg.refactor(None) == -1
+        y = he.solve(p.rhs[:, :1])
+        assert np.array_equal(hg.solve(p.rhs[:, :1]), y)
