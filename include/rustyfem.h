@@ -113,6 +113,16 @@ int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t *loc, const double *
                          const int64_t *indices, const double *data, int32_t leaf, int64_t *order_out,
                          double *info_out, int64_t *err_dof);
 
+/* Static hazard check of the GPU factorization schedule for this input
+ * (additive, host only): 0 when no tile of any launch writes an arena entry
+ * that another tile of the same launch reads or writes, 1 with a description
+ * in msg otherwise, -1 on an internal error.  plant = 1 adds a known race
+ * (an in-place write of each pivot block by its diagonal tile) so that tests
+ * can check that the check fires; plant = 0 checks the real schedule. */
+int32_t rustyfem_check_plan(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
+                            const int64_t *indices, const double *data, int32_t leaf, int32_t plant, char *msg,
+                            size_t msg_len);
+
 #ifdef __cplusplus
 }
 #endif

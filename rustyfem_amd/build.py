@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "librustyfem_ffi.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HOST_SOURCES = ["symbolic.cpp", "mf_plan.cpp", "ffi.cpp"]
+HOST_SOURCES = ["symbolic.cpp", "mf_plan.cpp", "plan_check.cpp", "ffi.cpp"]
 HIP_SOURCES = ["mf_gpu.hip", "indef_gpu.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]

@@ -62,6 +62,7 @@ struct Handle {
   std::vector<int64_t> owner;
   std::unique_ptr<rf::GpuFactor> gpu;
   std::unique_ptr<rf::IndefGpu> ind;
+  std::unique_ptr<rf::Symbolic> sym;  // kept only by host-only analyses
 };
 
 namespace {
@@ -188,7 +189,10 @@ Handle* build(size_t n, size_t nn, const int64_t* loc, const double* coords, con
       h->vmap[p] = vi;
     }
   }
-  if (!gpu) return h.release();
+  if (!gpu) {
+    h->sym = std::make_unique<rf::Symbolic>(std::move(sym));
+    return h.release();
+  }
   if (mode == Mode::Pd) {
     rf::Plan plan = rf::build_plan(sym, h->colptr, h->rowidx);
     h->nlevels = static_cast<int64_t>(plan.levels.size());
@@ -394,6 +398,31 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
+
+int32_t rustyfem_check_plan(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
+                            const int64_t* indices, const double* data, int32_t leaf, int32_t plant, char* msg,
+                            size_t msg_len) {
+  set_error("");
+  try {
+    Options opt;
+    opt.leaf = leaf > 0 ? leaf : 16;
+    int64_t e = -1;
+    std::unique_ptr<Handle> h(build(n, nn, loc, coords, indptr, indices, data, Mode::Pd, opt, &e, false));
+    if (!h) throw std::runtime_error("input rejected before the factorization");
+    rf::Plan plan = rf::build_plan(*h->sym, h->colptr, h->rowidx);
+    const std::string r = rf::check_plan(plan, plant);
+    if (msg && msg_len) {
+      std::strncpy(msg, r.c_str(), msg_len - 1);
+      msg[msg_len - 1] = 0;
+    }
+    return r.empty() ? 0 : 1;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
 
 int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
                          const int64_t* indices, const double* data, int32_t leaf, int64_t* order_out,

@@ -49,8 +49,8 @@ struct DiagTask { int32_t front, j0, jb, tile0, ntiles, pad; };
 struct TrsmTile { int32_t front, j0, jb, r0, task, pad; };
 // one inner step of one front for the rows [r0, r0 + TRS): left-looking update
 // from the outer block's columns [J0, j0), LDL^T of the diagonal block and
-// TRSM; the tile with r0 == j0 is the diagonal block itself and writes L_BB,
-// D and the block inverse (slot `task`)
+// TRSM; the tile with r0 == j0 is the diagonal block itself and writes D and
+// the block inverse (slot `task`)
 struct PanelTile { int32_t front, J0, j0, jb, r0, task; };
 // solve row tile below pivot block `task`; `next` is the front's following
 // block (-1 if none), `local`/`ntiles` its position among the block's tiles
@@ -127,5 +127,11 @@ struct Plan {
 
 // Builds the plan from the symbolic analysis and the Lower pattern (internal numbering).
 Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const std::vector<int64_t>& rowidx);
+
+// Static hazard check: "" if no tile of a launch writes what another tile of
+// the same launch reads or writes, else a description of the first conflict.
+// plant = 1 adds an in-place write of each pivot block by its diagonal tile
+// (a known race) so that a test can see the check fire.
+std::string check_plan(const Plan& P, int plant = 0);
 
 }  // namespace rf

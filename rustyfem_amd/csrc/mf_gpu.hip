@@ -246,8 +246,9 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 //  2. LDL^T of the 64 x 64 diagonal block and its inverse L^-1, in the
 //     registers of wave 0 (lane i holds row i; rows are broadcast with
 //     readlane), with the pivot test of src/sparse.rs:528-531;
-//  3. the diagonal tile writes L_BB, D and L^-1; every other tile applies
-//     L[r, B] = A_R L_BB^-T D^-1 to its rows (FP64 MFMA) and writes them.
+//  3. the diagonal tile writes D and L^-1 (the arena's pivot block is left as
+//     assembled, because the other tiles of the launch read it); every other
+//     tile applies L[r, B] = A_R L_BB^-T D^-1 to its rows (FP64 MFMA).
 // Every tile factors the (identical) diagonal block itself, so one launch
 // covers the whole inner step.
 template <typename T>
@@ -359,12 +360,11 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
   }
   __syncthreads();
   if (diag_tile) {
+    // The pivot block itself stays as assembled in the arena: the other tiles
+    // of this launch read it, and everything after the factorization uses the
+    // block's inverse (V) and D instead of L_BB.
     double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-      const int i = idx % NB, q = idx / NB;
-      vb[idx] = Vs[q][i];
-      if (i < jb && q < jb && i > q) F[t.j0 + i + static_cast<int64_t>(t.j0 + q) * m] = T(Ad[q][i]);
-    }
+    for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
     if (tid < jb) dpiv[d.start + t.j0 + tid] = Ad[tid][tid];
     return;
   }

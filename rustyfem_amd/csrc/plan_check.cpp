@@ -1,0 +1,115 @@
+// Static hazard check of a factorization plan: inside one launch, no tile may
+// write an arena element that another tile of the same launch reads or
+// writes.  Footprints are rectangles (front, rows, columns) of the column-major
+// fronts; the extend-add footprints are sets of parent entries reached
+// through the child->parent row map.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "mf.hpp"
+
+namespace rf {
+
+namespace {
+
+struct Rect {
+  int32_t front;
+  int32_t r0, r1, c0, c1;  // half-open
+  bool write;
+  int32_t tile;
+};
+
+bool overlap(const Rect& a, const Rect& b) {
+  return a.front == b.front && a.r0 < b.r1 && b.r0 < a.r1 && a.c0 < b.c1 && b.c0 < a.c1;
+}
+
+std::string describe(const char* what, const Rect& a, const Rect& b) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: tile %d %s front %d rows [%d,%d) cols [%d,%d) vs tile %d %s rows [%d,%d) cols [%d,%d)",
+           what, a.tile, a.write ? "writes" : "reads", a.front, a.r0, a.r1, a.c0, a.c1, b.tile,
+           b.write ? "writes" : "reads", b.r0, b.r1, b.c0, b.c1);
+  return buf;
+}
+
+// pairwise check of the rectangles of one launch (grouped by front)
+std::string check_rects(std::vector<Rect>& v, const char* what) {
+  std::sort(v.begin(), v.end(), [](const Rect& a, const Rect& b) { return a.front < b.front; });
+  size_t i = 0;
+  while (i < v.size()) {
+    size_t j = i;
+    while (j < v.size() && v[j].front == v[i].front) ++j;
+    for (size_t a = i; a < j; ++a) {
+      if (!v[a].write) continue;
+      for (size_t b = i; b < j; ++b)
+        if (b != a && v[b].tile != v[a].tile && overlap(v[a], v[b])) return describe(what, v[a], v[b]);
+    }
+    i = j;
+  }
+  return "";
+}
+
+}  // namespace
+
+std::string check_plan(const Plan& P, int plant) {
+  for (size_t L = 0; L < P.levels.size(); ++L) {
+    const LevelPlan& lp = P.levels[L];
+    // extend-add: each launch writes parent entries (map[x], map[y]); they must be distinct
+    for (int s = 0; s < 2; ++s) {
+      std::vector<std::pair<int64_t, int32_t>> w;  // (arena offset, tile)
+      for (int64_t t = lp.ea[s].begin; t < lp.ea[s].begin + lp.ea[s].count; ++t) {
+        const EaTile& e = P.ea[t];
+        const FrontDesc& c = P.fd[e.child];
+        const FrontDesc& p = P.fd[c.parent];
+        const int32_t* cm = P.cmap.data() + c.cmap_off;
+        for (int x = e.x0; x < std::min(e.x0 + EA_T, c.u); ++x)
+          for (int y = e.y0; y < std::min(e.y0 + EA_T, x + 1); ++y)
+            w.push_back({p.off + cm[x] + static_cast<int64_t>(cm[y]) * p.m, static_cast<int32_t>(t)});
+      }
+      std::sort(w.begin(), w.end());
+      for (size_t i = 1; i < w.size(); ++i)
+        if (w[i].first == w[i - 1].first) {
+          char buf[160];
+          snprintf(buf, sizeof buf, "extend-add level %zu slot %d: tiles %d and %d write arena entry %lld", L, s,
+                   w[i - 1].second, w[i].second, static_cast<long long>(w[i].first));
+          return buf;
+        }
+    }
+    for (const Phase& ph : lp.phases) {
+      std::vector<Rect> v;
+      if (ph.kind == PhaseKind::Panel) {
+        for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
+          const PanelTile& pt = P.panel[t];
+          const int32_t j1 = pt.j0 + pt.jb, f = pt.front, m = P.fd[f].m, id = static_cast<int32_t>(t);
+          const int32_t rend = std::min(pt.r0 + TRS, m);
+          v.push_back({f, pt.j0, j1, pt.J0, pt.j0, false, id});   // Y: rows of the block, earlier columns
+          v.push_back({f, pt.j0, j1, pt.j0, j1, false, id});      // the pivot block as assembled
+          if (plant == 1 && pt.r0 == pt.j0) v.push_back({f, pt.j0, j1, pt.j0, j1, true, id});  // planted: L_BB in place
+          if (pt.r0 != pt.j0) {
+            v.push_back({f, pt.r0, rend, pt.J0, pt.j0, false, id});  // X: the tile's rows, earlier columns
+            v.push_back({f, pt.r0, rend, pt.j0, j1, false, id});     // A_R
+            v.push_back({f, pt.r0, rend, pt.j0, j1, true, id});      // L_R
+          }
+        }
+      } else if (ph.kind == PhaseKind::Gemm64 || ph.kind == PhaseKind::Gemm128) {
+        const int tn = ph.kind == PhaseKind::Gemm64 ? 64 : 128;
+        for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
+          const GemmTile& g = P.gemm[t];
+          const int32_t m = P.fd[g.front].m, id = static_cast<int32_t>(t);
+          const int32_t rend = std::min(g.r0 + GT_M, m), cend = std::min(g.c0 + tn, g.cend);
+          v.push_back({g.front, g.r0, rend, g.k0, g.k1, false, id});
+          v.push_back({g.front, g.c0, cend, g.k0, g.k1, false, id});
+          v.push_back({g.front, g.r0, rend, g.c0, cend, true, id});
+        }
+      } else {
+        continue;
+      }
+      const std::string e = check_rects(v, ph.kind == PhaseKind::Panel ? "panel" : "gemm");
+      if (!e.empty()) return "level " + std::to_string(L) + " " + e;
+    }
+  }
+  return "";
+}
+
+}  // namespace rf

@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(HERE, "librustyfem_ffi.so")
 
 EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
-           "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse"]
+           "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
+           "rustyfem_check_plan"]
 
 _lib = None
 
@@ -67,6 +68,8 @@ def lib():
         L.rustyfem_last_error.argtypes = []
         L.rustyfem_analyse.restype = C.c_int32
         L.rustyfem_analyse.argtypes = fac + [C.c_int32, vp, vp, i64p]
+        L.rustyfem_check_plan.restype = C.c_int32
+        L.rustyfem_check_plan.argtypes = fac + [C.c_int32, C.c_int32, C.c_char_p, sz]
         _lib = L
     return _lib
 
@@ -85,6 +88,16 @@ def analyse(p, leaf=16):
     if rc < 0:
         raise RuntimeError("rustyfem_analyse failed: " + last_error())
     return order[:int(p.n)], info, int(err.value)
+
+
+def check_plan(p, leaf=16, plant=0):
+    """Static hazard check of the GPU schedule (rustyfem_check_plan): '' when clean."""
+    keep, args = _arrays(p)
+    buf = C.create_string_buffer(512)
+    rc = lib().rustyfem_check_plan(*args, int(leaf), int(plant), buf, 512)
+    if rc < 0:
+        raise RuntimeError("rustyfem_check_plan failed: " + last_error())
+    return buf.value.decode()
 
 
 def _arrays(p):

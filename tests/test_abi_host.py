@@ -91,3 +91,19 @@ def test_leaf_option(rf, ora, leaf):
         assert np.array_equal(info, ho.info())
     finally:
         lib.ora_set_ffi_leaf(16)
+
+
+@pytest.mark.parametrize("case", ["hex9", "hex14", "cantilever", "plate"])
+def test_schedule_has_no_intra_launch_hazards(rf, case):
+    """No tile of a launch writes what another tile of the same launch reads or
+    writes (extend-add, fused panel and GEMM launches)."""
+    p = {"hex9": lambda: meshes.hex_block(9), "hex14": lambda: meshes.hex_block(14),
+         "cantilever": meshes.cantilever_cps4, "plate": lambda: meshes.plate_cpe4(120, 12, 1e11, 0.3)}[case]()
+    assert rf.check_plan(p) == ""
+
+
+def test_schedule_check_catches_planted_race(rf):
+    """The check fires on a planted race: the pivot block written in place by its
+    diagonal tile while the row tiles of the same launch read it."""
+    msg = rf.check_plan(meshes.hex_block(8), plant=1)
+    assert msg.startswith("level") and "panel" in msg
