@@ -104,6 +104,20 @@ void rustyfem_stats(const Handle *h, double *out, size_t len);
 /* Last error message of the calling thread ("" if none) (additive). */
 const char *rustyfem_last_error(void);
 
+/* Pivot sequence of a factored handle (additive, for parity checks), in the
+ * front postorder of src/sparse/indef.rs:236-253 and src/sparse/lu.rs:216-241:
+ * internal equation, d, e and kind of each pivot.  LDL^T: kind 1 = 1x1 pivot
+ * d, kind 2 = first of a 2x2 block [d_i e_i; e_i d_i+1], kind 0 = its second.
+ * LU: d = the U diagonal, kind 1, or kind 3 with e = the row permutation entry
+ * inside the partial-pivoting block of a root.  Positive definite: equation
+ * order, D, kind 1.  Returns the number of pivots (n_sym), or -1. */
+int64_t rustyfem_pivots(const Handle *h, int64_t *eq, double *d, double *e, int32_t *kind);
+
+/* [neg, zero, pos, delayed pivots, 2x2 blocks, factor entries] of a factored
+ * handle (additive); the inertia here excludes uncoupled DOFs (the fields of
+ * IndefFactor and LuFactor in src/sparse/indef.rs:57-70, src/sparse/lu.rs:100-106). */
+void rustyfem_factor_stats(const Handle *h, int64_t *out6);
+
 /* Host-only symbolic analysis of the positive definite path (additive): the
  * ordering rustyfem_order would report (n entries) and rustyfem_info's
  * [flops, factor entries], without touching the GPU.  0 on success, 1 when

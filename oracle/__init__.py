@@ -73,6 +73,9 @@ def lib():
             "ora_ffi_order": (None, [vp, i64p]),
             "ora_ffi_free": (None, [vp]),
             "ora_ffi_sym": (vp, [vp]),
+            "ora_ffi_pivots": (sz, [vp, i64p, f64p, f64p, vp]),
+            "ora_ffi_stats": (None, [vp, i64p]),
+            "ora_indef_last_forced": (C.c_int64, []),
             "ora_ffi_leaf": (sz, []),
             "ora_set_ffi_leaf": (None, [sz]),
             "ora_solve_lower": (C.c_int64, [sz, i64p, i64p, f64p, f64p, C.c_int, sz, f64p, i64p, C.c_double,
@@ -306,6 +309,21 @@ class Handle:
     def info(self):
         o = np.zeros(2)
         lib().ora_ffi_info(self._h, o)
+        return o
+
+    def pivots(self):
+        """(equation, d, e, kind) of every pivot in front postorder."""
+        eq = np.zeros(max(self.n, 1), dtype=np.int64)
+        d = np.zeros(max(self.n, 1))
+        e = np.zeros(max(self.n, 1))
+        kind = np.zeros(max(self.n, 1), dtype=np.int32)
+        c = lib().ora_ffi_pivots(self._h, eq, d, e, kind.ctypes.data)
+        return eq[:c], d[:c], e[:c], kind[:c]
+
+    def stats(self):
+        """neg, zero, pos, delayed, two_by_two, factor_size."""
+        o = np.zeros(6, dtype=np.int64)
+        lib().ora_ffi_stats(self._h, o)
         return o
 
     def order(self):

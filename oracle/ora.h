@@ -71,6 +71,11 @@ void ora_ifactor_free(ora_ifactor *f);
 void ora_ifactor_solve(const ora_ifactor *f, double *x);
 /* out: neg, zero, pos, delayed, two_by_two, factor_size */
 void ora_ifactor_stats(const ora_ifactor *f, int64_t *out6);
+/* pivot sequence in front postorder: equation, d, e (2x2 off-diagonal on the
+   first of a pair), kind (1 = 1x1, 2 = first of a 2x2, 0 = second); returns the count */
+/* candidates the roots left to Bunch-Kaufman in the last factorization (test probe) */
+int64_t ora_indef_last_forced(void);
+size_t ora_ifactor_pivots(const ora_ifactor *f, int64_t *eq, double *d, double *e, int32_t *kind);
 
 /* ---- multifrontal LU (src/sparse/lu.rs) ---- */
 typedef struct ora_lufactor ora_lufactor;
@@ -81,6 +86,9 @@ void ora_lufactor_free(ora_lufactor *f);
 void ora_lufactor_solve(const ora_lufactor *f, double *x);
 /* out: delayed, factor_size */
 void ora_lufactor_stats(const ora_lufactor *f, int64_t *out2);
+/* pivot sequence in front postorder: equation, U diagonal, kind 1 (threshold
+   pivot) or 3 (partial pivoting at a root; e = the row permutation entry) */
+size_t ora_lufactor_pivots(const ora_lufactor *f, int64_t *eq, double *d, double *e, int32_t *kind);
 
 /* ---- the C ABI of ffi/src/lib.rs, restated with an ora_ prefix ---- */
 typedef struct ora_handle ora_handle;
@@ -97,6 +105,10 @@ int32_t ora_ffi_solve(const ora_handle *h, size_t nrhs, double *b);
 void ora_ffi_info(const ora_handle *h, double *out);
 void ora_ffi_order(const ora_handle *h, int64_t *out);
 void ora_ffi_free(ora_handle *h);
+/* pivot sequence of the handle's factor (PD: equation order, d, kind 1); returns the count */
+size_t ora_ffi_pivots(const ora_handle *h, int64_t *eq, double *d, double *e, int32_t *kind);
+/* neg, zero, pos, delayed, two_by_two, factor_size (PD: 0, 0, n, 0, 0, size) */
+void ora_ffi_stats(const ora_handle *h, int64_t *out6);
 /* the symbolic analysis a handle holds (NULL-safe), for structure checks */
 const ora_sym *ora_ffi_sym(const ora_handle *h);
 /* leaf size used by the C ABI (ffi/src/lib.rs:17) */

@@ -289,4 +289,31 @@ void ora_ffi_order(const ora_handle *h, int64_t *out) {
 
 void ora_ffi_free(ora_handle *h) { handle_free(h); }
 
+size_t ora_ffi_pivots(const ora_handle *h, int64_t *eq, double *d, double *e, int32_t *kind) {
+  if (h->ind) return ora_ifactor_pivots(h->ind, eq, d, e, kind);
+  if (h->lu) return ora_lufactor_pivots(h->lu, eq, d, e, kind);
+  ora_factor_pivots(h->pd, d);
+  for (size_t i = 0; i < h->n_sym; i++) {
+    eq[i] = (int64_t)i;
+    e[i] = 0.0;
+    kind[i] = 1;
+  }
+  return h->n_sym;
+}
+
+void ora_ffi_stats(const ora_handle *h, int64_t *out6) {
+  memset(out6, 0, 6 * sizeof(int64_t));
+  if (h->ind) {
+    ora_ifactor_stats(h->ind, out6);
+  } else if (h->lu) {
+    int64_t st[2];
+    ora_lufactor_stats(h->lu, st);
+    out6[3] = st[0];
+    out6[5] = st[1];
+  } else {
+    out6[2] = (int64_t)h->n_sym;
+    out6[5] = (int64_t)h->factor_size;
+  }
+}
+
 const ora_sym *ora_ffi_sym(const ora_handle *h) { return h ? h->sym : NULL; }

@@ -263,6 +263,10 @@ typedef struct {
   vi kind;
 } dek;
 
+/* candidates left to the Bunch-Kaufman loop of a root in the last factorization (a test probe) */
+static int64_t g_forced = 0;
+int64_t ora_indef_last_forced(void) { return g_forced; }
+
 /* src/sparse/indef.rs:416-500; returns npiv or -(1+equation) on a singular root */
 static int64_t eliminate(double *f, size_t m, size_t kf, int64_t *fs, const ictx *c, int root, int threads, dek *o) {
   size_t p = 0, width = IND_KB;
@@ -312,6 +316,7 @@ static int64_t eliminate(double *f, size_t m, size_t kf, int64_t *fs, const ictx
     }
   }
   if (p < kf && root) {
+    g_forced += (int64_t)(kf - p);
     while (p < kf) {
       int kd = 0;
       size_t j = 0, r = 0;
@@ -509,6 +514,7 @@ static void inode(ijob *job) {
 ora_ifactor *ora_factor_indef(const ora_sym *S, const int64_t *colptr, const int64_t *rowidx, const double *val,
                               double threshold, int64_t *err) {
   *err = -1;
+  g_forced = 0;
   double thr = threshold < 1e-8 ? 1e-8 : (threshold > 0.5 ? 0.5 : threshold);
   ictx c = {S, colptr, rowidx, val, column_scale(colptr, rowidx, val, S->n), thr};
   ora_ifactor *F = (ora_ifactor *)ora_xcalloc(1, sizeof(ora_ifactor));
@@ -585,6 +591,20 @@ void ora_ifactor_stats(const ora_ifactor *F, int64_t *out6) {
   out6[3] = F->delayed;
   out6[4] = F->two;
   out6[5] = (int64_t)fsz;
+}
+
+size_t ora_ifactor_pivots(const ora_ifactor *F, int64_t *eq, double *d, double *e, int32_t *kind) {
+  size_t c = 0;
+  for (size_t fi = 0; fi < F->nf; fi++) {
+    const ifront *fr = &F->fr[fi];
+    for (size_t i = 0; i < fr->piv.n; i++, c++) {
+      eq[c] = fr->piv.a[i];
+      d[c] = fr->d[i];
+      e[c] = fr->e[i];
+      kind[c] = fr->kind[i];
+    }
+  }
+  return c;
 }
 
 /* src/sparse/indef.rs:536-591 */

@@ -401,6 +401,20 @@ void ora_lufactor_stats(const ora_lufactor *F, int64_t *out2) {
   out2[1] = (int64_t)tot;
 }
 
+size_t ora_lufactor_pivots(const ora_lufactor *F, int64_t *eq, double *d, double *e, int32_t *kind) {
+  size_t c = 0;
+  for (size_t fi = 0; fi < F->nf; fi++) {
+    const lufront *fr = &F->fr[fi];
+    for (size_t i = 0; i < fr->k; i++, c++) {
+      eq[c] = fr->piv.a[i];
+      d[c] = fr->lu11[i * fr->k + i];
+      e[c] = fr->rperm.n ? (double)fr->rperm.a[i] : 0.0;
+      kind[c] = fr->rperm.n ? 3 : 1;
+    }
+  }
+  return c;
+}
+
 /* src/sparse/lu.rs:369-404 */
 void ora_lufactor_solve(const ora_lufactor *F, double *x) {
   double *xp = NULL, *xr = NULL;

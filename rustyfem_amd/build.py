@@ -20,6 +20,8 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HOST_SOURCES = ["symbolic.cpp", "mf_plan.cpp", "plan_check.cpp", "ffi.cpp"]
 HIP_SOURCES = ["mf_gpu.hip", "indef_gpu.hip"]
+# kernels that restate a CPU operation order entry by entry: no contraction
+NO_CONTRACT = {"indef_gpu.hip"}
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
@@ -42,7 +44,8 @@ def build(verbose: bool = False) -> str:
         jobs.append((["g++"] + CXXFLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
     for s in HIP_SOURCES:
         o = os.path.join(OUT, s + ".o")
-        jobs.append(([HIPCC] + HIPFLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
+        extra = ["-ffp-contract=off"] if s in NO_CONTRACT else []
+        jobs.append(([HIPCC] + HIPFLAGS + extra + ["-c", os.path.join(SRC, s), "-o", o], o))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for msg in ex.map(lambda j: _run(j[0]), jobs):
             if verbose and msg:

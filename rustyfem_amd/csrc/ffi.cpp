@@ -399,6 +399,35 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
 
+int64_t rustyfem_pivots(const Handle* h, int64_t* eq, double* d, double* e, int32_t* kind) {
+  try {
+    if (h->ind) return h->ind->pivots(eq, d, e, kind);
+    if (!h->gpu) return -1;
+    h->gpu->pivots(d);
+    for (int64_t i = 0; i < h->n_sym; ++i) {
+      eq[i] = i;
+      e[i] = 0.0;
+      kind[i] = 1;
+    }
+    return h->n_sym;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_factor_stats(const Handle* h, int64_t* out) {
+  if (h->ind) {
+    h->ind->stats(out);
+    return;
+  }
+  out[0] = out[1] = out[3] = out[4] = 0;
+  out[2] = h->n_sym;
+  out[5] = h->factor_size;
+}
+
 int32_t rustyfem_check_plan(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
                             const int64_t* indices, const double* data, int32_t leaf, int32_t plant, char* msg,
                             size_t msg_len) {

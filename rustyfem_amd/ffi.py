@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "librustyfem_ffi.so")
 EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
            "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
-           "rustyfem_check_plan"]
+           "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats"]
 
 _lib = None
 
@@ -70,6 +70,10 @@ def lib():
         L.rustyfem_analyse.argtypes = fac + [C.c_int32, vp, vp, i64p]
         L.rustyfem_check_plan.restype = C.c_int32
         L.rustyfem_check_plan.argtypes = fac + [C.c_int32, C.c_int32, C.c_char_p, sz]
+        L.rustyfem_pivots.restype = C.c_int64
+        L.rustyfem_pivots.argtypes = [vp, vp, vp, vp, vp]
+        L.rustyfem_factor_stats.restype = None
+        L.rustyfem_factor_stats.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -194,6 +198,24 @@ class Handle:
         o = np.zeros(max(self.n, 1), dtype=np.int64)
         lib().rustyfem_order(self._h, o.ctypes.data)
         return o[:self.n]
+
+    def pivots(self):
+        """(equation, d, e, kind) of every pivot in front postorder."""
+        n = max(self.n, 1)
+        eq = np.zeros(n, dtype=np.int64)
+        d = np.zeros(n)
+        e = np.zeros(n)
+        kind = np.zeros(n, dtype=np.int32)
+        c = lib().rustyfem_pivots(self._h, eq.ctypes.data, d.ctypes.data, e.ctypes.data, kind.ctypes.data)
+        if c < 0:
+            raise RuntimeError(last_error())
+        return eq[:c], d[:c], e[:c], kind[:c]
+
+    def factor_stats(self):
+        """neg, zero, pos, delayed, two_by_two, factor_size."""
+        o = np.zeros(6, dtype=np.int64)
+        lib().rustyfem_factor_stats(self._h, o.ctypes.data)
+        return o
 
     def stats(self):
         o = np.zeros(13)

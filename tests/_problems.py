@@ -82,3 +82,50 @@ def laplacian_sigma(g):
     sigma = 0.5 * (lam[q] + lam[q + 1])
     below = sum(1 for l in lam if l < sigma)
     return sigma, below
+
+
+def _with_values(p, a):
+    a = a.tocsc()
+    a.sort_indices()
+    p.data = a.data.copy()
+    p.indptr = a.indptr.astype(np.int64)
+    p.indices = a.indices.astype(np.int64)
+    return p
+
+
+def bipartite_grid(g, seed, dscale):
+    """7-point grid graph with random +-[0.5, 1.5] couplings and a diagonal of
+    dscale * N(0, 1): strongly indefinite, exercises 2x2 and delayed pivots
+    (zero diagonal on an odd grid is singular)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    p = laplacian3d(g, 0.0)
+    a = sp.csc_matrix((p.data, p.indices, p.indptr), shape=(p.n, p.n)).tocoo()
+    w = {}
+    for r, c in zip(a.row, a.col):
+        if r < c:
+            w[(r, c)] = rng.uniform(0.5, 1.5) * (1.0 if rng.random() < 0.5 else -1.0)
+    vals = [dscale * rng.standard_normal() if r == c else w[(min(r, c), max(r, c))] for r, c in zip(a.row, a.col)]
+    return _with_values(p, sp.coo_matrix((vals, (a.row, a.col)), shape=(p.n, p.n)))
+
+
+def unsym_grid(g, seed, zero_frac=0.0, conv=0.6):
+    """Convection-diffusion-like unsymmetric values on the 7-point grid pattern;
+    a fraction of zero diagonals forces delayed pivots (all zero: the roots
+    take partial-pivoting LU)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    p = laplacian3d(g, 0.0)
+    a = sp.csc_matrix((p.data, p.indices, p.indptr), shape=(p.n, p.n)).tocoo()
+    vals = []
+    for r, c in zip(a.row, a.col):
+        if r == c:
+            vals.append(0.0 if rng.random() < zero_frac else 6.0 + rng.uniform(-1, 1))
+        else:
+            vals.append(-1.0 + conv * (1.0 if r > c else -1.0) + 0.2 * rng.standard_normal())
+    return _with_values(p, sp.coo_matrix((vals, (a.row, a.col)), shape=(p.n, p.n)))
+
+
+def csc(p):
+    import scipy.sparse as sp
+    return sp.csc_matrix((p.data, p.indices, p.indptr), shape=(p.n, p.n))
