@@ -113,6 +113,50 @@ const char *rustyfem_last_error(void);
  * order, D, kind 1.  Returns the number of pivots (n_sym), or -1. */
 int64_t rustyfem_pivots(const Handle *h, int64_t *eq, double *d, double *e, int32_t *kind);
 
+/* ------------------------------------------------ iterative path (additive) */
+
+/* Options of the GPU preconditioned Krylov solver. */
+typedef struct rustyfem_pcg_options {
+  int32_t krylov;       /* 0: PCG as src/solver/iterative/csr.rs:341-380; 1: flexible CG (FCG(1)) */
+  int32_t vcycle_fp32;  /* AMG V-cycle arithmetic: 0 FP64, 1 FP32 (the Krylov loop stays FP64) */
+  int32_t smoother;     /* 0: Chebyshev on [rho/ratio, 1.1 rho] of D^-1 A; 1: damped Jacobi,
+                           omega = 4/(3 rho) (src/solver/iterative/amg.rs:300) */
+  int32_t degree;       /* smoothing steps before and after the coarse correction; 0 = 2 */
+  double cheb_ratio;    /* 0 = 20 */
+} rustyfem_pcg_options;
+
+typedef struct PcgHandle PcgHandle;
+
+/* Preconditioner setup for the lower triangle (CSC, natural numbering: node v
+ * owns ndof[v] consecutive equations) -- the cached-preconditioner half of
+ * IterMatrix (src/solver/iterative.rs:208-243).  kind: 0 AMG (smoothed
+ * aggregation with rigid body modes, src/solver/iterative/amg.rs:282-341;
+ * the hierarchy is built on the host, the spectral radii on the GPU),
+ * 2 Jacobi.  NULL on error (rustyfem_last_error). */
+PcgHandle *rustyfem_pcg_setup(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,
+                              int32_t kind, size_t nn, const double *coords, const int64_t *ndof,
+                              const rustyfem_pcg_options *opt);
+
+/* Solves from x0 = 0 until the recursive residual satisfies ||r|| <= tol ||b||
+ * (src/solver/iterative/csr.rs:341-380).  b and x are host pointers, or device
+ * pointers when on_device != 0.  maxit <= 0 means max(20000, n).  Returns the
+ * iterations (-1 on an internal error). */
+int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol, int64_t maxit, int32_t on_device,
+                           double *rel_residual, int32_t *converged);
+
+/* [0] setup ms, [1] last solve ms, [2] levels, [3] operator complexity,
+ * [4] ms of one fine-matrix product, [5] its bytes, [6] coarsest size,
+ * [7] iterations of the last solve, [8..19] rows per level. */
+void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
+void rustyfem_pcg_free(PcgHandle *h);
+
+/* One-shot setup + solve with the default options: mirrors
+ * solve_lower(k, b, kind, coords, ndof, tol) -> (x, PcgReport)
+ * (src/solver/iterative.rs:275-292). */
+int64_t rustyfem_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,
+                             const double *b, int32_t kind, size_t nn, const double *coords, const int64_t *ndof,
+                             double tol, int64_t maxit, double *x, double *rel_residual, int32_t *converged);
+
 /* [neg, zero, pos, delayed pivots, 2x2 blocks, factor entries] of a factored
  * handle (additive); the inertia here excludes uncoupled DOFs (the fields of
  * IndefFactor and LuFactor in src/sparse/indef.rs:57-70, src/sparse/lu.rs:100-106). */

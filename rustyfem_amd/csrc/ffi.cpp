@@ -10,6 +10,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -18,6 +19,7 @@
 #include "gpu.hpp"
 #include "indef.hpp"
 #include "mf.hpp"
+#include "pcg.hpp"
 #include "symbolic.hpp"
 
 namespace {
@@ -63,6 +65,10 @@ struct Handle {
   std::unique_ptr<rf::GpuFactor> gpu;
   std::unique_ptr<rf::IndefGpu> ind;
   std::unique_ptr<rf::Symbolic> sym;  // kept only by host-only analyses
+  // solves share the handle's device work buffers and graphs: concurrent
+  // rustyfem_solve calls on one handle (allowed by ffi/src/lib.rs:336-371)
+  // are serialized here
+  mutable std::mutex solve_mu;
 };
 
 namespace {
@@ -319,6 +325,7 @@ int32_t rustyfem_inertia(const Handle* h, int64_t* out) {
 int32_t rustyfem_solve(const Handle* h, size_t nrhs, double* b) {
   if (h->n == 0 || nrhs == 0) return 0;
   try {
+    std::lock_guard<std::mutex> lock(h->solve_mu);
     if (h->mode == Mode::Pd)
       h->gpu->solve(b, static_cast<int>(nrhs), false, h->opt.refine_tol, h->opt.max_refine);
     else
@@ -366,6 +373,7 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
   if (h->n == 0 || nrhs == 0) return 0;
   try {
     if (h->mode != Mode::Pd) throw std::runtime_error("device solves need a positive definite handle");
+    std::lock_guard<std::mutex> lock(h->solve_mu);
     h->gpu->solve(d_b, static_cast<int>(nrhs), true, h->opt.refine_tol, h->opt.max_refine);
     return 0;
   } catch (const std::exception& ex) {
@@ -398,6 +406,79 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
+
+struct PcgHandle {
+  std::unique_ptr<rf::PcgGpu> solver;
+  int64_t n = 0;
+  std::mutex mu;
+};
+
+PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int32_t kind,
+                              size_t nn, const double* coords, const int64_t* ndof, const rustyfem_pcg_options* opt) {
+  set_error("");
+  try {
+    // the lower triangle in CSC, natural numbering (src/solver/iterative.rs:74-97)
+    for (size_t c = 0; c < n; ++c) {
+      if (colptr[c + 1] < colptr[c]) throw std::runtime_error("colptr is not monotone");
+      for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p)
+        if (rowidx[p] < static_cast<int64_t>(c) || rowidx[p] >= static_cast<int64_t>(n))
+          throw std::runtime_error("rowidx outside the lower triangle");
+    }
+    int64_t tot = 0;
+    for (size_t v = 0; v < nn; ++v) {
+      if (ndof[v] < 0) throw std::runtime_error("negative ndof");
+      tot += ndof[v];
+    }
+    if (kind == 0 && tot != static_cast<int64_t>(n)) throw std::runtime_error("sum of ndof differs from n");
+    rf::PcgOptions o;
+    if (opt) {
+      o.krylov = opt->krylov;
+      o.vcycle_fp32 = opt->vcycle_fp32;
+      o.smoother = opt->smoother;
+      if (opt->degree > 0) o.degree = opt->degree;
+      if (opt->cheb_ratio > 1.0) o.cheb_ratio = opt->cheb_ratio;
+    }
+    auto h = std::make_unique<PcgHandle>();
+    h->n = static_cast<int64_t>(n);
+    h->solver = rf::make_pcg_gpu(static_cast<int64_t>(n), colptr, rowidx, val, kind, static_cast<int64_t>(nn), coords,
+                                 ndof, o);
+    return h.release();
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return nullptr;
+}
+
+int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol, int64_t maxit, int32_t on_device,
+                           double* rel_residual, int32_t* converged) {
+  try {
+    std::lock_guard<std::mutex> lock(h->mu);
+    if (maxit <= 0) maxit = std::max<int64_t>(20000, h->n);
+    const rf::PcgResult r = h->solver->solve(b, x, tol, maxit, on_device != 0);
+    if (rel_residual) *rel_residual = r.rel_residual;
+    if (converged) *converged = r.converged ? 1 : 0;
+    return r.iterations;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) { h->solver->stats(out, len); }
+
+void rustyfem_pcg_free(PcgHandle* h) { delete h; }
+
+int64_t rustyfem_solve_lower(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
+                             const double* b, int32_t kind, size_t nn, const double* coords, const int64_t* ndof,
+                             double tol, int64_t maxit, double* x, double* rel_residual, int32_t* converged) {
+  std::unique_ptr<PcgHandle> h(rustyfem_pcg_setup(n, colptr, rowidx, val, kind, nn, coords, ndof, nullptr));
+  if (!h) return -1;
+  return rustyfem_pcg_solve(h.get(), b, x, tol, maxit, 0, rel_residual, converged);
+}
 
 int64_t rustyfem_pivots(const Handle* h, int64_t* eq, double* d, double* e, int32_t* kind) {
   try {

@@ -1,0 +1,47 @@
+// GPU preconditioned Krylov solver for the lower-triangle matrices of the
+// iterative path (src/solver/iterative.rs:275-292, csr.rs:326-380): AMG
+// V-cycle (Chebyshev or damped-Jacobi smoothing, FP64 or FP32), Jacobi;
+// standard PCG or flexible CG.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace rf {
+
+struct PcgOptions {
+  int krylov = 0;        // 0: PCG (csr.rs:341-380); 1: flexible CG (FCG(1), Notay)
+  int vcycle_fp32 = 0;   // V-cycle arithmetic: 0 FP64, 1 FP32 (matrices and vectors)
+  int smoother = 0;      // 0: Chebyshev of degree `degree`; 1: damped Jacobi (omega = 4 / (3 rho))
+  int degree = 2;        // Chebyshev degree, pre and post
+  double cheb_ratio = 20.0;  // Chebyshev interval [rho / ratio, rho]
+  int batch = 8;         // iterations between convergence checks on the host
+};
+
+struct PcgResult {
+  int64_t iterations = 0;
+  double rel_residual = 0.0;
+  bool converged = false;
+};
+
+class PcgGpu {
+ public:
+  virtual ~PcgGpu() = default;
+  // x = A^-1 b from x0 = 0; b and x in host or device memory
+  virtual PcgResult solve(const double* b, double* x, double tol, int64_t maxit, bool on_device) = 0;
+  // [0] setup ms (host + GPU), [1] last solve ms, [2] levels (incl. coarsest),
+  // [3] operator complexity, [4] fine-matrix SpMV ms (one product, measured),
+  // [5] fine-matrix bytes per SpMV, [6] coarsest size, [7] iterations of the last solve,
+  // [8..19] rows per level (finest first)
+  virtual void stats(double* out, size_t len) const = 0;
+  virtual std::vector<int64_t> level_sizes() const = 0;
+};
+
+// kind: 0 AMG (smoothed aggregation, rigid body modes), 2 Jacobi.  coords /
+// ndof describe the node blocks (natural numbering: node v owns ndof[v]
+// consecutive equations).  Throws when the AMG setup breaks down.
+std::unique_ptr<PcgGpu> make_pcg_gpu(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
+                                     int kind, int64_t nn, const double* coords, const int64_t* ndof,
+                                     const PcgOptions& opt);
+
+}  // namespace rf

@@ -1,0 +1,929 @@
+// GPU preconditioned Krylov solver (iterative path) on one MI355X.
+//
+// Matrices live in HBM in block-row form: the finest level as BSR with the
+// node's DOF count as block size (3x3 for 3D solids; a block row is one
+// node), coarser levels and the transfer operators as scalar CSR.  Every
+// sparse product is one group of G lanes per block row (values streamed
+// once, x gathered through the caches): HBM-bound.
+//
+// The Krylov loop (src/solver/iterative/csr.rs:341-380 for PCG; FCG(1) as
+// the flexible variant) keeps its scalars in device memory: one iteration is
+// a fixed sequence of kernels (SpMV fused with the p.q / p.r partial sums,
+// x/r update fused with r.r, preconditioner, z.r or z.q, p update) captured
+// once into a HIP graph and replayed; every kernel returns at once after the
+// convergence flag is set, so the host only checks the flag every few
+// iterations and the iteration count and x are those of the first iteration
+// that met the tolerance.  Reductions are deterministic (fixed partial slots
+// summed in a fixed order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "amg.hpp"
+#include "pcg.hpp"
+
+namespace rf {
+
+#define PHIPCHK(expr)                                                                                \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess) {                                                                          \
+      char b_[512];                                                                                  \
+      snprintf(b_, sizeof b_, "HIP error %s at %s:%d: %s", hipGetErrorString(e_), __FILE__, __LINE__, #expr); \
+      throw std::runtime_error(b_);                                                                  \
+    }                                                                                                \
+  } while (0)
+
+namespace {
+
+constexpr int RB = 256;       // threads per block
+constexpr int NPART = 1024;   // partial-sum slots (= blocks of the reduction kernels)
+
+struct Scal {
+  double bn, tol;
+  double rz, pq, pr, alpha, beta, rel;
+  int64_t it, maxit;
+  int32_t done, converged;
+  int32_t krylov, pad;
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__device__ __forceinline__ double sum_partials(const double* part, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < NPART; i += RB) s += part[i];
+  return block_sum(s, sh);
+}
+
+// ---------------------------------------------------------------------------
+// block-row sparse products: MODE 0 y = A x, 1 y = b - A x, 2 y += A x
+
+template <typename T, int BS, int G, int MODE>
+__global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, const int64_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                             const T* __restrict__ x, T* y, const T* b, const int32_t* done) {
+  if (done && *done) return;
+  const int64_t g = (static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
+  if (g >= nbr) return;
+  T s[BS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i) s[i] = T(0);
+  for (int64_t q = ptr[g] + lane; q < ptr[g + 1]; q += G) {
+    const int64_t c = idx[q];
+    const T* v = val + q * (BS * BS);
+    T xv[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) xv[j] = x[c * BS + j];
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+      for (int j = 0; j < BS; ++j) s[i] += v[i * BS + j] * xv[j];
+  }
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+    for (int o = G / 2; o > 0; o >>= 1) s[i] += __shfl_xor(s[i], o, G);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+      const int64_t r = g * BS + i;
+      if (MODE == 0) y[r] = s[i];
+      else if (MODE == 1) y[r] = b[r] - s[i];
+      else y[r] += s[i];
+    }
+  }
+}
+
+// Krylov product q = A p (FP64) with the partial sums of p.q and p.r
+template <int BS, int G>
+__global__ __launch_bounds__(RB) void k_spmv_dots(int64_t nbr, const int64_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                  const double* __restrict__ p, double* q, const double* __restrict__ r,
+                                                  double* part, const Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  const int lane = threadIdx.x % G;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * RB / G;
+  double apq = 0.0, apr = 0.0;
+  for (int64_t g = (static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x) / G; g < nbr; g += stride) {
+    double s[BS];
+#pragma unroll
+    for (int i = 0; i < BS; ++i) s[i] = 0.0;
+    for (int64_t k = ptr[g] + lane; k < ptr[g + 1]; k += G) {
+      const int64_t c = idx[k];
+      const double* v = val + k * (BS * BS);
+      double xv[BS];
+#pragma unroll
+      for (int j = 0; j < BS; ++j) xv[j] = p[c * BS + j];
+#pragma unroll
+      for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) s[i] += v[i * BS + j] * xv[j];
+    }
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+      for (int o = G / 2; o > 0; o >>= 1) s[i] += __shfl_xor(s[i], o, G);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < BS; ++i) {
+        const int64_t row = g * BS + i;
+        q[row] = s[i];
+        apq += p[row] * s[i];
+        apr += p[row] * r[row];
+      }
+    }
+  }
+  const double a = block_sum(apq, sh);
+  const double b = block_sum(apr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[NPART + blockIdx.x] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Krylov vector steps and scalars (FP64)
+
+__global__ __launch_bounds__(RB) void k_init(int64_t n, const double* __restrict__ b, double* x, double* r,
+                                             double* part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB) {
+    const double v = b[i];
+    x[i] = 0.0;
+    r[i] = v;
+    s += v * v;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_fin_bn(const double* part, Scal* sc) {
+  __shared__ double sh[4];
+  const double s = sum_partials(part, sh);
+  if (threadIdx.x == 0) sc->bn = sqrt(s);
+}
+
+// partial sums of a.b and c.d
+__global__ __launch_bounds__(RB) void k_dots(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                             const double* __restrict__ c, const double* __restrict__ d, double* part,
+                                             const Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  double s = 0.0, t = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB) {
+    s += a[i] * b[i];
+    t += c[i] * d[i];
+  }
+  s = block_sum(s, sh);
+  t = block_sum(t, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    part[NPART + blockIdx.x] = t;
+  }
+}
+
+// first direction: rz = r.z (PCG) and p = z
+__global__ __launch_bounds__(RB) void k_fin_rz0(const double* part, Scal* sc) {
+  __shared__ double sh[4];
+  const double s = sum_partials(part, sh);
+  if (threadIdx.x == 0) sc->rz = s;
+}
+
+__global__ __launch_bounds__(RB) void k_copy(int64_t n, const double* __restrict__ a, double* b, const Scal* sc) {
+  if (sc && sc->done) return;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    b[i] = a[i];
+}
+
+// alpha = rz / pq (PCG) or pr / pq (FCG); breakdown if pq <= 0
+__global__ __launch_bounds__(RB) void k_fin_alpha(const double* part, Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  const double pq = sum_partials(part, sh);
+  const double pr = sum_partials(part + NPART, sh);
+  if (threadIdx.x == 0) {
+    sc->pq = pq;
+    sc->pr = pr;
+    if (!(pq > 0.0)) {
+      sc->it += 1;
+      sc->done = 1;
+      sc->converged = 0;
+    } else {
+      sc->alpha = (sc->krylov ? pr : sc->rz) / pq;
+    }
+  }
+}
+
+// x += alpha p, r -= alpha q, partial r.r
+__global__ __launch_bounds__(RB) void k_update_xr(int64_t n, double* x, double* r, const double* __restrict__ p,
+                                                  const double* __restrict__ q, double* part, const Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  const double a = sc->alpha;
+  double s = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB) {
+    x[i] += a * p[i];
+    const double ri = r[i] - a * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// relative residual of the recursion, convergence and the iteration limit
+__global__ __launch_bounds__(RB) void k_fin_rel(const double* part, Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  const double s = sum_partials(part, sh);
+  if (threadIdx.x == 0) {
+    const double rel = sqrt(s) / sc->bn;
+    sc->rel = rel;
+    sc->it += 1;
+    if (rel <= sc->tol) {
+      sc->done = 1;
+      sc->converged = 1;
+    } else if (sc->it >= sc->maxit) {
+      sc->done = 1;
+      sc->converged = 0;
+    }
+  }
+}
+
+// beta: PCG rz_new / rz (csr.rs:370-373); FCG(1) -z.q / p.q
+__global__ __launch_bounds__(RB) void k_fin_beta(const double* part, Scal* sc) {
+  __shared__ double sh[4];
+  if (sc->done) return;
+  const double s = sum_partials(part, sh);
+  if (threadIdx.x == 0) {
+    if (sc->krylov) {
+      sc->beta = -s / sc->pq;
+    } else {
+      sc->beta = s / sc->rz;
+      sc->rz = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_update_p(int64_t n, double* p, const double* __restrict__ z, const Scal* sc) {
+  if (sc->done) return;
+  const double b = sc->beta;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    p[i] = z[i] + b * p[i];
+}
+
+__global__ __launch_bounds__(RB) void k_jacobi(int64_t n, const double* __restrict__ dinv, const double* __restrict__ r,
+                                               double* z, const Scal* sc) {
+  if (sc->done) return;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    z[i] = r[i] * dinv[i];
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle pieces (T = float or double)
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(RB) void k_convert(int64_t n, const TI* __restrict__ a, TO* b, const int32_t* done) {
+  if (done && *done) return;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    b[i] = static_cast<TO>(a[i]);
+}
+
+// first smoothing step from x = 0: d = c0 D^-1 b, x = d
+template <typename T>
+__global__ __launch_bounds__(RB) void k_smooth0(int64_t n, const T* __restrict__ dinv, const T* __restrict__ b, T* x, T* d,
+                                                T c0, const int32_t* done) {
+  if (*done) return;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB) {
+    const T v = c0 * dinv[i] * b[i];
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
+// d = c1 d + c2 D^-1 r, x += d
+template <typename T>
+__global__ __launch_bounds__(RB) void k_smooth(int64_t n, const T* __restrict__ dinv, const T* __restrict__ r, T* x, T* d,
+                                               T c1, T c2, const int32_t* done) {
+  if (*done) return;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB) {
+    const T v = c1 * d[i] + c2 * dinv[i] * r[i];
+    d[i] = v;
+    x[i] += v;
+  }
+}
+
+// dense coarse solve x = M b, one wave per row
+template <typename T>
+__global__ __launch_bounds__(RB) void k_gemv(int64_t n, const T* __restrict__ m, const T* __restrict__ b, T* x,
+                                             const int32_t* done) {
+  if (*done) return;
+  const int64_t row = (static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T* mr = m + row * n;
+  T s = T(0);
+  for (int64_t j = lane; j < n; j += 64) s += mr[j] * b[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[row] = s;
+}
+
+// power-method helpers (src/solver/iterative/amg.rs:260-276), FP64
+__global__ __launch_bounds__(RB) void k_pm_init(int64_t n, double* x) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    x[i] = 1.0 + static_cast<double>((i * 7919) % 13) / 13.0;
+}
+
+__global__ __launch_bounds__(RB) void k_sumsq(int64_t n, const double* __restrict__ x, double* part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    s += x[i] * x[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_fin_sqrt(const double* part, double* out) {
+  __shared__ double sh[4];
+  const double s = sum_partials(part, sh);
+  if (threadIdx.x == 0) *out = sqrt(s);
+}
+
+__global__ __launch_bounds__(RB) void k_pm_scale(int64_t n, double* x, const double* nrm) {
+  const double s = *nrm;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    x[i] /= s;
+}
+
+__global__ __launch_bounds__(RB) void k_pm_divdiag(int64_t n, double* y, const double* __restrict__ diag) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
+    y[i] /= diag[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t n) {
+    release();
+    if (n) PHIPCHK(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+DBuf upload(const std::vector<T>& v) {
+  DBuf b;
+  b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty()) PHIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return b;
+}
+
+unsigned grid_for(int64_t threads) {
+  const int64_t b = (threads + RB - 1) / RB;
+  return static_cast<unsigned>(std::max<int64_t>(1, b));
+}
+
+unsigned grid_stride(int64_t n) { return static_cast<unsigned>(std::min<int64_t>(NPART, std::max<int64_t>(1, (n + RB - 1) / RB))); }
+
+// a block-row matrix on the device, values in FP64 and/or FP32
+struct DevMat {
+  int64_t nbr = 0, nbc = 0;  // block rows / block columns
+  int bs = 1;
+  int group = 8;             // lanes per block row
+  int64_t nnzb = 0;
+  DBuf ptr, idx, v64, v32;
+
+  size_t bytes_per_product(bool fp32) const {
+    return static_cast<size_t>(nnzb) * (bs * bs * (fp32 ? 4 : 8) + 4) + static_cast<size_t>(nbr + 1) * 8 +
+           static_cast<size_t>(nbr + nbc) * bs * (fp32 ? 4 : 8);
+  }
+};
+
+int pick_group(double per_row) {
+  if (per_row <= 6) return 4;
+  if (per_row <= 12) return 8;
+  if (per_row <= 24) return 16;
+  return 32;
+}
+
+// scalar CSR -> block rows of size bs (every entry placed, missing ones zero)
+DevMat make_mat(const Csr& a, int bs, bool keep64, bool keep32) {
+  DevMat M;
+  M.bs = bs;
+  M.nbr = a.nrows / bs;
+  M.nbc = a.ncols / bs;
+  std::vector<int64_t> bptr(M.nbr + 1, 0);
+  std::vector<int32_t> bidx;
+  std::vector<double> bval;
+  std::vector<int64_t> mark(M.nbc, -1);
+  std::vector<int32_t> cols;
+  for (int64_t br = 0; br < M.nbr; ++br) {
+    cols.clear();
+    for (int i = 0; i < bs; ++i)
+      for (int64_t p = a.ptr[br * bs + i]; p < a.ptr[br * bs + i + 1]; ++p) {
+        const int32_t bc = a.idx[p] / bs;
+        if (mark[bc] != br) {
+          mark[bc] = br;
+          cols.push_back(bc);
+        }
+      }
+    std::sort(cols.begin(), cols.end());
+    const size_t base = bidx.size();
+    bidx.insert(bidx.end(), cols.begin(), cols.end());
+    bval.resize(bidx.size() * bs * bs, 0.0);
+    for (int i = 0; i < bs; ++i)
+      for (int64_t p = a.ptr[br * bs + i]; p < a.ptr[br * bs + i + 1]; ++p) {
+        const int32_t c = a.idx[p];
+        const size_t k = base + (std::lower_bound(cols.begin(), cols.end(), c / bs) - cols.begin());
+        bval[k * bs * bs + i * bs + (c % bs)] = a.val[p];
+      }
+    bptr[br + 1] = static_cast<int64_t>(bidx.size());
+  }
+  M.nnzb = static_cast<int64_t>(bidx.size());
+  M.group = pick_group(M.nbr ? static_cast<double>(M.nnzb) / M.nbr : 1.0);
+  M.ptr = upload(bptr);
+  M.idx = upload(bidx);
+  if (keep64) M.v64 = upload(bval);
+  if (keep32) {
+    std::vector<float> f(bval.begin(), bval.end());
+    M.v32 = upload(f);
+  }
+  return M;
+}
+
+template <typename T>
+const T* vals(const DevMat& M);
+template <>
+const double* vals<double>(const DevMat& M) { return M.v64.as<double>(); }
+template <>
+const float* vals<float>(const DevMat& M) { return M.v32.as<float>(); }
+
+template <typename T, int BS, int G>
+void launch_spmv_g(const DevMat& M, int mode, const T* x, T* y, const T* b, const int32_t* done, hipStream_t s) {
+  const unsigned grid = grid_for(M.nbr * G);
+  if (mode == 0)
+    k_spmv<T, BS, G, 0><<<grid, RB, 0, s>>>(M.nbr, M.ptr.as<int64_t>(), M.idx.as<int32_t>(), vals<T>(M), x, y, b, done);
+  else if (mode == 1)
+    k_spmv<T, BS, G, 1><<<grid, RB, 0, s>>>(M.nbr, M.ptr.as<int64_t>(), M.idx.as<int32_t>(), vals<T>(M), x, y, b, done);
+  else
+    k_spmv<T, BS, G, 2><<<grid, RB, 0, s>>>(M.nbr, M.ptr.as<int64_t>(), M.idx.as<int32_t>(), vals<T>(M), x, y, b, done);
+}
+
+template <typename T, int BS>
+void launch_spmv_bs(const DevMat& M, int mode, const T* x, T* y, const T* b, const int32_t* done, hipStream_t s) {
+  switch (M.group) {
+    case 4: launch_spmv_g<T, BS, 4>(M, mode, x, y, b, done, s); break;
+    case 8: launch_spmv_g<T, BS, 8>(M, mode, x, y, b, done, s); break;
+    case 16: launch_spmv_g<T, BS, 16>(M, mode, x, y, b, done, s); break;
+    default: launch_spmv_g<T, BS, 32>(M, mode, x, y, b, done, s); break;
+  }
+}
+
+template <typename T>
+void spmv(const DevMat& M, int mode, const T* x, T* y, const T* b, const int32_t* done, hipStream_t s) {
+  switch (M.bs) {
+    case 1: launch_spmv_bs<T, 1>(M, mode, x, y, b, done, s); break;
+    case 2: launch_spmv_bs<T, 2>(M, mode, x, y, b, done, s); break;
+    case 3: launch_spmv_bs<T, 3>(M, mode, x, y, b, done, s); break;
+    default: throw std::runtime_error("internal: unsupported block size");
+  }
+}
+
+template <int BS>
+void spmv_dots_bs(const DevMat& M, const double* p, double* q, const double* r, double* part, const Scal* sc,
+                  hipStream_t s) {
+  if (!M.v64.p) throw std::runtime_error("internal: FP64 values missing");
+  const auto* P = M.ptr.as<int64_t>();
+  const auto* I = M.idx.as<int32_t>();
+  const auto* V = M.v64.as<double>();
+  switch (M.group) {
+    case 4: k_spmv_dots<BS, 4><<<NPART, RB, 0, s>>>(M.nbr, P, I, V, p, q, r, part, sc); break;
+    case 8: k_spmv_dots<BS, 8><<<NPART, RB, 0, s>>>(M.nbr, P, I, V, p, q, r, part, sc); break;
+    case 16: k_spmv_dots<BS, 16><<<NPART, RB, 0, s>>>(M.nbr, P, I, V, p, q, r, part, sc); break;
+    default: k_spmv_dots<BS, 32><<<NPART, RB, 0, s>>>(M.nbr, P, I, V, p, q, r, part, sc); break;
+  }
+}
+
+void spmv_dots(const DevMat& M, const double* p, double* q, const double* r, double* part, const Scal* sc, hipStream_t s) {
+  switch (M.bs) {
+    case 1: spmv_dots_bs<1>(M, p, q, r, part, sc, s); break;
+    case 2: spmv_dots_bs<2>(M, p, q, r, part, sc, s); break;
+    case 3: spmv_dots_bs<3>(M, p, q, r, part, sc, s); break;
+    default: throw std::runtime_error("internal: unsupported block size");
+  }
+}
+
+template <typename T>
+struct VLevel {
+  int64_t n = 0;
+  const DevMat* A = nullptr;   // the level matrix (owned by the solver)
+  DevMat P, R;                 // transfer operators (not on the coarsest level)
+  DBuf dinv, x, b, r, d;
+  // smoothing: first step coefficient and per-step (c1, c2)
+  T c0 = 0;
+  std::vector<std::pair<T, T>> steps;
+};
+
+class PcgImpl final : public PcgGpu {
+ public:
+  PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int kind, int64_t nn,
+          const double* coords, const int64_t* ndof, const PcgOptions& opt);
+  ~PcgImpl() override {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  PcgResult solve(const double* b, double* x, double tol, int64_t maxit, bool on_device) override;
+  void stats(double* out, size_t len) const override {
+    std::vector<double> v(20, 0.0);
+    v[0] = setup_ms_;
+    v[1] = solve_ms_;
+    v[2] = static_cast<double>(sizes_.size());
+    v[3] = complexity_;
+    v[4] = spmv_ms_;
+    v[5] = static_cast<double>(A_.bytes_per_product(false));
+    v[6] = sizes_.empty() ? 0.0 : static_cast<double>(sizes_.back());
+    v[7] = static_cast<double>(last_it_);
+    for (size_t i = 0; i < sizes_.size() && i < 12; ++i) v[8 + i] = static_cast<double>(sizes_[i]);
+    for (size_t i = 0; i < len && i < v.size(); ++i) out[i] = v[i];
+  }
+  std::vector<int64_t> level_sizes() const override { return sizes_; }
+
+ private:
+  template <typename T>
+  void build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb);
+  template <typename T>
+  void vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done);
+  void precondition(const double* r, double* z, const int32_t* done);
+  void iteration();
+  double power_radius(const DevMat& M, const std::vector<double>& diag);
+
+  int64_t n_ = 0;
+  int kind_ = 0;
+  PcgOptions opt_;
+  hipStream_t stream_ = nullptr;
+  DevMat A_;                       // finest matrix, FP64 (+ FP32 for an FP32 V-cycle)
+  std::vector<DevMat> coarse_A_;   // coarser level matrices (V-cycle precision)
+  std::vector<VLevel<double>> v64_;
+  std::vector<VLevel<float>> v32_;
+  DBuf cm64_, cx64_, cb64_, cm32_, cx32_, cb32_;
+  DBuf jdinv_;
+  DBuf x_, r_, z_, p_, q_, part_, scal_;
+  DBuf b32_, x32_;
+  hipGraphExec_t graph_ = nullptr;
+  std::vector<int64_t> sizes_;
+  double complexity_ = 1.0, setup_ms_ = 0, solve_ms_ = 0, spmv_ms_ = 0;
+  int64_t last_it_ = 0;
+};
+
+double PcgImpl::power_radius(const DevMat& M, const std::vector<double>& diag) {
+  const int64_t n = M.nbr * M.bs;
+  DBuf x, y, d = upload(diag), part, nrm;
+  x.alloc(n * 8);
+  y.alloc(n * 8);
+  part.alloc(NPART * 8);
+  nrm.alloc(2 * 8);
+  const unsigned g = grid_stride(n);
+  k_pm_init<<<g, RB, 0, stream_>>>(n, x.as<double>());
+  double* px = x.as<double>();
+  double* py = y.as<double>();
+  for (int it = 0; it < 15; ++it) {
+    PHIPCHK(hipMemsetAsync(part.p, 0, NPART * 8, stream_));
+    k_sumsq<<<g, RB, 0, stream_>>>(n, px, part.as<double>());
+    k_fin_sqrt<<<1, RB, 0, stream_>>>(part.as<double>(), nrm.as<double>());
+    k_pm_scale<<<g, RB, 0, stream_>>>(n, px, nrm.as<double>());
+    spmv<double>(M, 0, px, py, nullptr, nullptr, stream_);
+    k_pm_divdiag<<<g, RB, 0, stream_>>>(n, py, d.as<double>());
+    PHIPCHK(hipMemsetAsync(part.p, 0, NPART * 8, stream_));
+    k_sumsq<<<g, RB, 0, stream_>>>(n, py, part.as<double>());
+    k_fin_sqrt<<<1, RB, 0, stream_>>>(part.as<double>(), nrm.as<double>() + 1);
+    std::swap(px, py);
+  }
+  double rho = 0.0;
+  PHIPCHK(hipMemcpyAsync(&rho, nrm.as<double>() + 1, 8, hipMemcpyDeviceToHost, stream_));
+  PHIPCHK(hipStreamSynchronize(stream_));
+  return rho * 1.05;
+}
+
+// Chebyshev coefficients for D^-1 A on [hi / ratio, hi] (three-term recurrence)
+template <typename T>
+void chebyshev(double hi, double ratio, int degree, T& c0, std::vector<std::pair<T, T>>& steps) {
+  const double lo = hi / ratio;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  c0 = static_cast<T>(1.0 / theta);
+  steps.clear();
+  for (int k = 1; k < degree; ++k) {
+    const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+    steps.push_back({static_cast<T>(rho_new * rho_old), static_cast<T>(2.0 * rho_new / delta)});
+    rho_old = rho_new;
+  }
+}
+
+template <typename T>
+void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb) {
+  const bool f32 = std::is_same<T, float>::value;
+  lv.resize(H.levels.size());
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    const AmgHostLevel& L = H.levels[l];
+    VLevel<T>& V = lv[l];
+    V.n = L.A.nrows;
+    V.A = l == 0 ? &A_ : &coarse_A_[l - 1];
+    V.P = make_mat(L.P, 1, !f32, f32);
+    V.R = make_mat(L.R, 1, !f32, f32);
+    std::vector<T> di(V.n);
+    for (int64_t i = 0; i < V.n; ++i) di[i] = static_cast<T>(1.0 / L.diag[i]);
+    V.dinv = upload(di);
+    V.x.alloc(V.n * sizeof(T));
+    V.b.alloc(V.n * sizeof(T));
+    V.r.alloc(V.n * sizeof(T));
+    V.d.alloc(V.n * sizeof(T));
+    if (opt_.smoother == 0) {
+      chebyshev<T>(1.1 * L.rho, opt_.cheb_ratio, std::max(1, opt_.degree), V.c0, V.steps);
+    } else {
+      V.c0 = static_cast<T>(L.omega);
+      V.steps.assign(std::max(1, opt_.degree) - 1, {T(0), static_cast<T>(L.omega)});
+    }
+  }
+  const int64_t nc = H.coarse.nrows;
+  const std::vector<double> minv = coarse_inverse(H.coarse_l, H.coarse_d, nc);
+  std::vector<T> mt(minv.begin(), minv.end());
+  cm = upload(mt);
+  cx.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
+  cb.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
+}
+
+PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int kind, int64_t nn,
+                 const double* coords, const int64_t* ndof, const PcgOptions& opt)
+    : n_(n), kind_(kind), opt_(opt) {
+  const auto t0 = std::chrono::steady_clock::now();
+  PHIPCHK(hipStreamCreate(&stream_));
+  if (kind != 0 && kind != 2) throw std::runtime_error("preconditioner kind not available on the GPU (0 AMG, 2 Jacobi)");
+  Csr a = csr_from_lower(n, colptr, rowidx, val);
+  // block size of the finest matrix: the common DOF count of the nodes
+  int bs = nn > 0 ? static_cast<int>(ndof[0]) : 1;
+  int64_t tot = 0;
+  for (int64_t v = 0; v < nn; ++v) {
+    tot += ndof[v];
+    if (ndof[v] != bs) bs = 1;
+  }
+  if (tot != n || bs < 1 || bs > 3) bs = 1;
+  const bool f32 = opt_.vcycle_fp32 != 0;
+  A_ = make_mat(a, bs, true, f32 && kind == 0);
+  sizes_.assign(1, n);
+  if (kind == 2) {
+    std::vector<double> d = csr_diag(a);
+    for (double& v : d) v = v != 0.0 ? 1.0 / v : 1.0;
+    jdinv_ = upload(d);
+  } else {
+    std::vector<int64_t> bptr(nn + 1, 0);
+    for (int64_t v = 0; v < nn; ++v) bptr[v + 1] = bptr[v] + ndof[v];
+    int nb = 0;
+    std::vector<double> bm = rigid_body_modes(nn, coords, ndof, nb);
+    int calls = 0;
+    auto radius = [&](const Csr& m, const std::vector<double>& diag) -> double {
+      if (calls++ == 0) return power_radius(A_, diag);  // the finest level is already on the device
+      DevMat tmp = make_mat(m, 1, true, false);
+      return power_radius(tmp, diag);
+    };
+    AmgHost H = amg_setup(std::move(a), std::move(bptr), std::move(bm), nb, radius);
+    sizes_ = H.sizes;
+    complexity_ = H.complexity;
+    // coarser level matrices in the V-cycle precision
+    coarse_A_.reserve(H.levels.size());
+    for (size_t l = 1; l < H.levels.size(); ++l) coarse_A_.push_back(make_mat(H.levels[l].A, 1, !f32, f32));
+    if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
+    else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
+  }
+  x_.alloc(n * 8);
+  r_.alloc(n * 8);
+  z_.alloc(n * 8);
+  p_.alloc(n * 8);
+  q_.alloc(n * 8);
+  part_.alloc(2 * NPART * 8);
+  scal_.alloc(sizeof(Scal));
+  if (f32 && kind == 0) {
+    b32_.alloc(n * 4);
+    x32_.alloc(n * 4);
+  }
+  PHIPCHK(hipStreamSynchronize(stream_));
+  setup_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T>
+void PcgImpl::vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done) {
+  hipStream_t s = stream_;
+  const bool f32 = std::is_same<T, float>::value;
+  DBuf& cm = f32 ? cm32_ : cm64_;
+  DBuf& cx = f32 ? cx32_ : cx64_;
+  DBuf& cb = f32 ? cb32_ : cb64_;
+  VLevel<T>& V = lv[l];
+  const unsigned g = grid_stride(V.n);
+  T* x = V.x.template as<T>();
+  T* b = V.b.template as<T>();
+  T* r = V.r.template as<T>();
+  T* d = V.d.template as<T>();
+  const T* dinv = V.dinv.template as<T>();
+  // pre-smoothing from x = 0
+  k_smooth0<T><<<g, RB, 0, s>>>(V.n, dinv, b, x, d, V.c0, done);
+  for (const auto& st : V.steps) {
+    spmv<T>(*V.A, 1, x, r, b, done, s);
+    k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, st.first, st.second, done);
+  }
+  // coarse-grid correction
+  spmv<T>(*V.A, 1, x, r, b, done, s);
+  const bool last = l + 1 == lv.size();
+  T* bc = last ? cb.template as<T>() : lv[l + 1].b.template as<T>();
+  T* xc = last ? cx.template as<T>() : lv[l + 1].x.template as<T>();
+  spmv<T>(V.R, 0, r, bc, nullptr, done, s);
+  if (last) {
+    const int64_t nc = sizes_.back();
+    k_gemv<T><<<grid_for(nc * 64), RB, 0, s>>>(nc, cm.template as<T>(), bc, xc, done);
+  } else {
+    vcycle<T>(lv, l + 1, done);
+  }
+  spmv<T>(V.P, 2, xc, x, nullptr, done, s);
+  // post-smoothing
+  spmv<T>(*V.A, 1, x, r, b, done, s);
+  k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, T(0), V.c0, done);
+  for (const auto& st : V.steps) {
+    spmv<T>(*V.A, 1, x, r, b, done, s);
+    k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, st.first, st.second, done);
+  }
+}
+
+void PcgImpl::precondition(const double* r, double* z, const int32_t* done) {
+  const Scal* sc = scal_.as<Scal>();
+  if (kind_ == 2) {
+    k_jacobi<<<grid_stride(n_), RB, 0, stream_>>>(n_, jdinv_.as<double>(), r, z, sc);
+    return;
+  }
+  const unsigned g = grid_stride(n_);
+  if (opt_.vcycle_fp32) {
+    float* b0 = v32_.empty() ? cb32_.as<float>() : v32_[0].b.as<float>();
+    float* x0 = v32_.empty() ? cx32_.as<float>() : v32_[0].x.as<float>();
+    k_convert<double, float><<<g, RB, 0, stream_>>>(n_, r, b0, done);
+    if (v32_.empty()) k_gemv<float><<<grid_for(n_ * 64), RB, 0, stream_>>>(n_, cm32_.as<float>(), b0, x0, done);
+    else vcycle<float>(v32_, 0, done);
+    k_convert<float, double><<<g, RB, 0, stream_>>>(n_, x0, z, done);
+  } else {
+    double* b0 = v64_.empty() ? cb64_.as<double>() : v64_[0].b.as<double>();
+    double* x0 = v64_.empty() ? cx64_.as<double>() : v64_[0].x.as<double>();
+    k_copy<<<g, RB, 0, stream_>>>(n_, r, b0, sc);
+    if (v64_.empty()) k_gemv<double><<<grid_for(n_ * 64), RB, 0, stream_>>>(n_, cm64_.as<double>(), b0, x0, done);
+    else vcycle<double>(v64_, 0, done);
+    k_copy<<<g, RB, 0, stream_>>>(n_, x0, z, sc);
+  }
+}
+
+// one Krylov iteration (csr.rs:356-377)
+void PcgImpl::iteration() {
+  Scal* sc = scal_.as<Scal>();
+  const int32_t* done = &sc->done;
+  double* part = part_.as<double>();
+  const unsigned g = grid_stride(n_);
+  spmv_dots(A_, p_.as<double>(), q_.as<double>(), r_.as<double>(), part, sc, stream_);
+  k_fin_alpha<<<1, RB, 0, stream_>>>(part, sc);
+  k_update_xr<<<NPART, RB, 0, stream_>>>(n_, x_.as<double>(), r_.as<double>(), p_.as<double>(), q_.as<double>(), part, sc);
+  k_fin_rel<<<1, RB, 0, stream_>>>(part, sc);
+  precondition(r_.as<double>(), z_.as<double>(), done);
+  if (opt_.krylov) k_dots<<<NPART, RB, 0, stream_>>>(n_, z_.as<double>(), q_.as<double>(), z_.as<double>(), q_.as<double>(), part, sc);
+  else k_dots<<<NPART, RB, 0, stream_>>>(n_, r_.as<double>(), z_.as<double>(), r_.as<double>(), z_.as<double>(), part, sc);
+  k_fin_beta<<<1, RB, 0, stream_>>>(part, sc);
+  k_update_p<<<g, RB, 0, stream_>>>(n_, p_.as<double>(), z_.as<double>(), sc);
+}
+
+PcgResult PcgImpl::solve(const double* b, double* x, double tol, int64_t maxit, bool on_device) {
+  const auto t0 = std::chrono::steady_clock::now();
+  PcgResult res;
+  if (n_ == 0) {
+    res.converged = true;
+    return res;
+  }
+  DBuf bh;
+  const double* db = b;
+  if (!on_device) {
+    bh.alloc(n_ * 8);
+    PHIPCHK(hipMemcpyAsync(bh.p, b, n_ * 8, hipMemcpyHostToDevice, stream_));
+    db = bh.as<double>();
+  }
+  Scal s0{};
+  s0.tol = tol;
+  s0.maxit = maxit;
+  s0.krylov = opt_.krylov;
+  Scal* sc = scal_.as<Scal>();
+  PHIPCHK(hipMemcpyAsync(sc, &s0, sizeof s0, hipMemcpyHostToDevice, stream_));
+  double* part = part_.as<double>();
+  k_init<<<NPART, RB, 0, stream_>>>(n_, db, x_.as<double>(), r_.as<double>(), part);
+  k_fin_bn<<<1, RB, 0, stream_>>>(part, sc);
+  Scal sh{};
+  PHIPCHK(hipMemcpyAsync(&sh, sc, sizeof sh, hipMemcpyDeviceToHost, stream_));
+  PHIPCHK(hipStreamSynchronize(stream_));
+  if (sh.bn == 0.0) {
+    // x_ was cleared by k_init
+    PHIPCHK(hipMemcpyAsync(x, x_.p, n_ * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
+    PHIPCHK(hipStreamSynchronize(stream_));
+    res.converged = true;
+    last_it_ = 0;
+    return res;
+  }
+  // z = M r, rz = r.z, p = z
+  precondition(r_.as<double>(), z_.as<double>(), &sc->done);
+  k_dots<<<NPART, RB, 0, stream_>>>(n_, r_.as<double>(), z_.as<double>(), r_.as<double>(), z_.as<double>(), part, sc);
+  k_fin_rz0<<<1, RB, 0, stream_>>>(part, sc);
+  k_copy<<<grid_stride(n_), RB, 0, stream_>>>(n_, z_.as<double>(), p_.as<double>(), sc);
+  if (!graph_) {
+    hipGraph_t g;
+    PHIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    iteration();
+    PHIPCHK(hipStreamEndCapture(stream_, &g));
+    PHIPCHK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+    PHIPCHK(hipGraphDestroy(g));
+  }
+  const int batch = std::max(1, opt_.batch);
+  int64_t launched = 0;
+  for (;;) {
+    for (int k = 0; k < batch; ++k) PHIPCHK(hipGraphLaunch(graph_, stream_));
+    launched += batch;
+    PHIPCHK(hipMemcpyAsync(&sh, sc, sizeof sh, hipMemcpyDeviceToHost, stream_));
+    PHIPCHK(hipStreamSynchronize(stream_));
+    if (sh.done) break;
+    if (launched > maxit + 2 * batch) throw std::runtime_error("internal: Krylov loop did not stop");
+  }
+  if (on_device) PHIPCHK(hipMemcpyAsync(x, x_.p, n_ * 8, hipMemcpyDeviceToDevice, stream_));
+  else PHIPCHK(hipMemcpyAsync(x, x_.p, n_ * 8, hipMemcpyDeviceToHost, stream_));
+  PHIPCHK(hipStreamSynchronize(stream_));
+  res.iterations = sh.it;
+  res.rel_residual = sh.rel;
+  res.converged = sh.converged != 0;
+  last_it_ = sh.it;
+  solve_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // one timed fine-matrix product (the HBM-bound kernel of the loop)
+  {
+    hipEvent_t e0, e1;
+    PHIPCHK(hipEventCreate(&e0));
+    PHIPCHK(hipEventCreate(&e1));
+    PHIPCHK(hipEventRecord(e0, stream_));
+    spmv<double>(A_, 0, p_.as<double>(), q_.as<double>(), nullptr, nullptr, stream_);
+    PHIPCHK(hipEventRecord(e1, stream_));
+    PHIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PHIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    spmv_ms_ = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return res;
+}
+
+}  // namespace
+
+std::unique_ptr<PcgGpu> make_pcg_gpu(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
+                                     int kind, int64_t nn, const double* coords, const int64_t* ndof,
+                                     const PcgOptions& opt) {
+  return std::make_unique<PcgImpl>(n, colptr, rowidx, val, kind, nn, coords, ndof, opt);
+}
+
+}  // namespace rf

@@ -20,9 +20,16 @@ LIB_PATH = os.path.join(HERE, "librustyfem_ffi.so")
 EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
            "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
-           "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats"]
+           "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats", "rustyfem_pcg_setup",
+           "rustyfem_pcg_solve", "rustyfem_pcg_stats", "rustyfem_pcg_free", "rustyfem_solve_lower"]
 
 _lib = None
+
+
+class PcgOptions(C.Structure):
+    """rustyfem_pcg_options (include/rustyfem.h)."""
+    _fields_ = [("krylov", C.c_int32), ("vcycle_fp32", C.c_int32), ("smoother", C.c_int32), ("degree", C.c_int32),
+                ("cheb_ratio", C.c_double)]
 
 
 class Options(C.Structure):
@@ -74,6 +81,17 @@ def lib():
         L.rustyfem_pivots.argtypes = [vp, vp, vp, vp, vp]
         L.rustyfem_factor_stats.restype = None
         L.rustyfem_factor_stats.argtypes = [vp, vp]
+        L.rustyfem_pcg_setup.restype = vp
+        L.rustyfem_pcg_setup.argtypes = [sz, vp, vp, vp, C.c_int32, sz, vp, vp, C.POINTER(PcgOptions)]
+        L.rustyfem_pcg_solve.restype = C.c_int64
+        L.rustyfem_pcg_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int64, C.c_int32, f64p, C.POINTER(C.c_int32)]
+        L.rustyfem_pcg_stats.restype = None
+        L.rustyfem_pcg_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_pcg_free.restype = None
+        L.rustyfem_pcg_free.argtypes = [vp]
+        L.rustyfem_solve_lower.restype = C.c_int64
+        L.rustyfem_solve_lower.argtypes = [sz, vp, vp, vp, vp, C.c_int32, sz, vp, vp, C.c_double, C.c_int64, vp, f64p,
+                                           C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -223,3 +241,62 @@ class Handle:
         keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
                 "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops"]
         return dict(zip(keys, o.tolist()))
+
+
+KIND = {"amg": 0, "ic0": 1, "jacobi": 2}
+
+
+class Pcg:
+    """GPU preconditioned Krylov solver on a lower-triangle CSC matrix in natural
+    numbering (rustyfem_pcg_setup / _solve)."""
+
+    def __init__(self, n, colptr, rowidx, val, kind="amg", coords=None, ndof=None, krylov="pcg", vcycle="fp64",
+                 smoother="chebyshev", degree=2, cheb_ratio=20.0):
+        self.n = int(n)
+        self._keep = [np.ascontiguousarray(colptr, dtype=np.int64), np.ascontiguousarray(rowidx, dtype=np.int64),
+                      np.ascontiguousarray(val, dtype=np.float64)]
+        if coords is None:
+            coords = np.zeros((self.n, 3))
+            ndof = np.ones(self.n, dtype=np.int64)
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        nd = np.ascontiguousarray(ndof, dtype=np.int64)
+        self._keep += [c, nd]
+        o = PcgOptions(1 if krylov == "fcg" else 0, 1 if vcycle == "fp32" else 0, 1 if smoother == "jacobi" else 0,
+                       int(degree), float(cheb_ratio))
+        self._h = lib().rustyfem_pcg_setup(self.n, self._keep[0].ctypes.data, self._keep[1].ctypes.data,
+                                           self._keep[2].ctypes.data, KIND[kind], c.shape[0], c.ctypes.data,
+                                           nd.ctypes.data, C.byref(o))
+        if not self._h:
+            raise RuntimeError(last_error())
+
+    def solve(self, b, tol=1e-10, maxit=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        rel = C.c_double(0.0)
+        conv = C.c_int32(0)
+        it = lib().rustyfem_pcg_solve(self._h, b.ctypes.data, x.ctypes.data, float(tol), int(maxit), 0, C.byref(rel),
+                                      C.byref(conv))
+        if it < 0:
+            raise RuntimeError(last_error())
+        return x, int(it), float(rel.value), bool(conv.value)
+
+    def solve_device(self, d_b, d_x, tol=1e-10, maxit=0):
+        rel = C.c_double(0.0)
+        conv = C.c_int32(0)
+        it = lib().rustyfem_pcg_solve(self._h, d_b, d_x, float(tol), int(maxit), 1, C.byref(rel), C.byref(conv))
+        if it < 0:
+            raise RuntimeError(last_error())
+        return int(it), float(rel.value), bool(conv.value)
+
+    def stats(self):
+        o = np.zeros(20)
+        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 20)
+        lv = int(o[2])
+        return {"setup_ms": o[0], "solve_ms": o[1], "levels": lv, "complexity": o[3], "spmv_ms": o[4],
+                "spmv_bytes": o[5], "coarse": int(o[6]), "iterations": int(o[7]),
+                "sizes": [int(v) for v in o[8:8 + min(lv, 12)]]}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().rustyfem_pcg_free(self._h)
+            self._h = None
