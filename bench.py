@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="cube edge of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied device-to-device twice before the run (a kernel of known traffic for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the multifrontal factorization of one problem is not sharded "
@@ -110,6 +113,15 @@ def main():
     import numpy as np
     from rustyfem_amd import ffi, hip, meshes
     ffi.lib()
+    if args.copy_calibration > 0:
+        cnt = args.copy_calibration * (1 << 20) // 8
+        src, dst = hip.DeviceArray(cnt), hip.DeviceArray(cnt)
+        for _ in range(2):
+            dst.copy_from(src)
+        hip.synchronize()
+        src.free()
+        dst.free()
+        log(f"copy calibration: 2 x {args.copy_calibration} MiB device-to-device")
     t0 = time.perf_counter()
     p = meshes.hex_block(args.size, nrhs=args.nrhs)
     log(f"problem {p.name}: n={p.n} nnz={p.nnz} nrhs={args.nrhs} built in {time.perf_counter() - t0:.1f}s")
