@@ -92,6 +92,154 @@ def cpu_baseline(size, work_full, nrhs):
                        f"{rel:.1e}; value = full-workload flops / sample rate (+ solve scaled by nnz(L))")}
 
 
+def lower_of(p):
+    import numpy as np
+    import scipy.sparse as sp
+    a = p.to_scipy().tocoo()
+    m = a.row >= a.col
+    low = sp.csc_matrix((a.data[m], (a.row[m], a.col[m])), shape=a.shape)
+    low.sort_indices()
+    return low.indptr.astype(np.int64), low.indices.astype(np.int64), low.data.copy()
+
+
+def run_amg(args):
+    """The heterogeneous-stiffness block (C3D8 B-bar, lognormal E with log-std 1.5,
+    correlation length 0.1, contrast clipped to 1e3, clamped base, body load):
+    AMG preconditioned Krylov to ||r|| <= 1e-8 ||b||, FP32 V-cycle inside FP64 FCG
+    vs all-FP64 PCG.  One step = one solve from x0 = 0 with the preconditioner
+    already set up and b in HBM."""
+    import numpy as np
+    from rustyfem_amd import ffi, hip, meshes
+    ffi.lib()
+    t0 = time.perf_counter()
+    p = meshes.hetero_block(args.size, seed=4)
+    _, coords, ndof, _, _ = meshes.node_graph(p)
+    cp, ri, v = lower_of(p)
+    log(f"problem hetero {args.size}^3: n={p.n} built in {time.perf_counter() - t0:.1f}s")
+    b = np.ascontiguousarray(p.rhs[:, 0])
+    d_b = hip.DeviceArray.from_host(b)
+    tol = 1e-8
+    res = {}
+    for name, kw in (("fp32_fcg", {"krylov": "fcg", "vcycle": "fp32"}), ("fp64_pcg", {"krylov": "pcg", "vcycle": "fp64"})):
+        t0 = time.perf_counter()
+        s = ffi.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, **kw)
+        st = s.stats()
+        log(f"{name}: setup {time.perf_counter() - t0:.1f}s levels {st['sizes']} complexity {st['complexity']:.3f}")
+        d_x = hip.DeviceArray(p.n)
+        for i in range(args.warmup):
+            s.solve_device(d_b.ptr, d_x.ptr, tol=tol)
+        times = []
+        for i in range(args.steps):
+            hip.synchronize()
+            t1 = time.perf_counter()
+            it, rel, conv = s.solve_device(d_b.ptr, d_x.ptr, tol=tol)
+            hip.synchronize()
+            times.append(time.perf_counter() - t1)
+            log(f"{name}: step {i} {times[-1] * 1e3:.1f} ms iterations {it} rel {rel:.2e} converged {conv}")
+        x = d_x.download()
+        true_rel = float(np.linalg.norm(b - p.to_scipy() @ x) / np.linalg.norm(b))
+        st = s.stats()
+        res[name] = {"time_s": float(np.median(times)), "iterations": it, "converged": bool(conv),
+                     "rel_residual": rel, "true_rel_residual": true_rel, "setup_s": st["setup_ms"] * 1e-3,
+                     "levels": st["sizes"], "complexity": st["complexity"], "spmv_ms": st["spmv_ms"],
+                     "spmv_bytes": st["spmv_bytes"], "solution": x}
+        d_x.free()
+        del s
+    a, f = res["fp32_fcg"], res["fp64_pcg"]
+    achieved = a["spmv_bytes"] / (a["spmv_ms"] * 1e-3) / 1e9
+    err = float(np.max(np.abs(a["solution"] - f["solution"])) / np.max(np.abs(f["solution"])))
+    for r in res.values():
+        r.pop("solution")
+    out = {"metric": "AMG-Krylov solve time to ||r||/||b|| <= 1e-8, FP32 V-cycle in FP64 FCG vs all-FP64 PCG",
+           "value": a["time_s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": a["time_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32+fp64", "data": "synthetic",
+           "config": {"workload": f"heterogeneous C3D8 B-bar block {args.size}^3 elements (n={p.n}), lognormal E "
+                                  f"(log-std 1.5, corr. length 0.1, contrast 1e3), clamped base, body load 1e4 N/m^3",
+                      "n": p.n},
+           "fp32_fcg": a, "fp64_pcg": f, "speedup_fp32_vs_fp64": f["time_s"] / a["time_s"],
+           "fp32_vs_fp64_displacement_rel_error": err,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                        "frac": achieved / PEAK["hbm_gbs"], "traffic": None,
+                        "kernel": "k_spmv<double,3,G> (BSR 3x3 fine-matrix product, FP64)"}}
+    print(json.dumps(out), flush=True)
+
+
+def run_batch(args):
+    """512 independent CPE4 plates (nx ~ U{20..200}, ny = nx/10, E ~ U[70e9, 210e9],
+    nu ~ U[0.25, 0.35], 8 edge-traction load cases each, seed 3), factored and
+    solved together.  One step = refactor of every plate from HBM + the 8-RHS
+    solve of every plate + FP64 refinement check to 1e-10."""
+    import numpy as np
+    from rustyfem_amd import ffi, hip, meshes
+    ffi.lib()
+    t0 = time.perf_counter()
+    plates = meshes.plate_batch(count=args.batch_count, seed=3, nrhs=8)
+    ntot = sum(p.n for p in plates)
+    log(f"{len(plates)} plates, {ntot} DOF, built in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    b = ffi.Batch(plates, max_refine=30, refine_tol=1e-10)
+    if not b.ok:
+        raise RuntimeError(f"batch factorization failed: {ffi.last_error()} {b.err}")
+    info = b.info()
+    log(f"analysis + first factorization {time.perf_counter() - t0:.1f}s  {info}")
+    allb = np.concatenate([p.rhs for p in plates], axis=0)
+    d_b = hip.DeviceArray.from_host(np.asfortranarray(allb).reshape(-1, order="F"))
+    d_x = hip.DeviceArray(d_b.count)
+
+    def step():
+        rc, _ = b.refactor(None)
+        if rc != 0:
+            raise RuntimeError("batch refactor failed")
+        d_x.copy_from(d_b)
+        b.solve_device(d_x.ptr, 8)
+
+    for _ in range(args.warmup):
+        step()
+    times = []
+    for i in range(args.steps):
+        hip.synchronize()
+        t1 = time.perf_counter()
+        step()
+        hip.synchronize()
+        times.append(time.perf_counter() - t1)
+        st = b.stats()
+        log(f"step {i} {times[-1] * 1e3:.1f} ms factor {st['factor_ms']:.1f} solve {st['solve_ms']:.1f} "
+            f"refine steps {st['refine_steps']:.0f} rel {st['rel_residual']:.2e}")
+    x = d_x.download().reshape(allb.shape, order="F")
+    worst = 0.0
+    off = 0
+    for p in plates:
+        xi = x[off:off + p.n]
+        off += p.n
+        worst = max(worst, float(np.max(np.linalg.norm(p.rhs - p.to_scipy() @ xi, axis=0) /
+                                        np.linalg.norm(p.rhs, axis=0))))
+    t = float(np.median(times))
+    out = {"metric": "batched factor + 8-RHS solve of 512 plates to ||r||/||b|| <= 1e-10 (FP64, refined)",
+           "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": f"{len(plates)} CPE4 plates, nx~U{{20..200}}, ny=nx/10, 8 load cases, seed 3; "
+                                  f"one forest, level-batched launches; {ntot} DOF, {info['fronts']} fronts, "
+                                  f"{info['levels']} levels", "n": ntot, "work": info["work"]},
+           "problems_per_s": len(plates) / t, "true_rel_residual_max": worst, "stats": b.stats()}
+    if args.cpu_sample > 0:
+        import oracle
+        oracle.build()
+        sample = plates[:args.cpu_sample]
+        t1 = time.perf_counter()
+        wsum = 0.0
+        for p in sample:
+            h, _, _ = oracle.Handle.factor(p)
+            h.solve(p.rhs)
+            wsum += h.info()[0]
+        dt = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": dt * info["work"] / wsum, "unit": "s", "cores": int(oracle.threads()),
+                               "kind": "port",
+                               "sample": f"C oracle factor + 8-RHS solve of the first {len(sample)} plates "
+                                         f"({dt:.2f} s, {wsum:.3e} flops), scaled to the batch by flops"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +249,10 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="cube edge of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--workload", choices=["direct", "amg", "batch"], default="direct",
+                    help="direct: the multifrontal block (default); amg: the heterogeneous block with AMG-Krylov; "
+                         "batch: 512 plates factored and solved together")
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied device-to-device twice before the run (a kernel of known traffic for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -109,6 +261,12 @@ def main():
         print("bench.py runs on one GPU: the multifrontal factorization of one problem is not sharded "
               "across GPUs in this build (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "amg":
+        run_amg(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
+        return
     os.environ.setdefault("RUSTYFEM_KERNEL_TIMING", "1")
     import numpy as np
     from rustyfem_amd import ffi, hip, meshes

@@ -113,6 +113,38 @@ const char *rustyfem_last_error(void);
  * order, D, kind 1.  Returns the number of pivots (n_sym), or -1. */
 int64_t rustyfem_pivots(const Handle *h, int64_t *eq, double *d, double *e, int32_t *kind);
 
+/* ------------------------------------------------ batches of problems (additive) */
+
+typedef struct BatchHandle BatchHandle;
+
+/* Factors nprob independent positive definite problems together (arrays of
+ * per-problem inputs with the meaning of rustyfem_factor's arguments).  Each
+ * problem is ordered as rustyfem_factor orders it alone; the elimination trees
+ * form one forest, so every GPU launch covers the fronts of all problems at
+ * the same tree level.  err_dof (nprob entries) is -1 or the caller DOF of
+ * the failing pivot / invalid input (-1 with NULL for invalid loc/indices).
+ * NULL on any failure.  No reference counterpart: the reference factors one
+ * problem per call. */
+BatchHandle *rustyfem_factor_batch(size_t nprob, const size_t *n, const size_t *nn, const int64_t *const *loc,
+                                   const double *const *coords, const int64_t *const *indptr,
+                                   const int64_t *const *indices, const double *const *data,
+                                   const rustyfem_options *opt, int64_t *err_dof);
+/* New values (data != NULL) or the values in device memory (data == NULL);
+ * 0, 1 on a failing pivot (err_dof set), -1 on an internal error. */
+int32_t rustyfem_refactor_batch(BatchHandle *h, const double *const *data, int64_t *err_dof);
+/* Solves every problem in place: b[i] holds nrhs columns of problem i. */
+int32_t rustyfem_solve_batch(const BatchHandle *h, size_t nrhs, double *const *b);
+/* Device variant: d_b is the concatenation of all problems' DOFs, column-major
+ * (sum n) x nrhs. */
+int32_t rustyfem_solve_batch_device(const BatchHandle *h, size_t nrhs, double *d_b);
+/* [problems, total DOFs, fronts, levels, work flops] */
+void rustyfem_batch_info(const BatchHandle *h, double *out5);
+/* as rustyfem_stats */
+void rustyfem_batch_stats(const BatchHandle *h, double *out, size_t len);
+/* rustyfem_order of problem i */
+void rustyfem_batch_order(const BatchHandle *h, size_t i, int64_t *out);
+void rustyfem_free_batch(BatchHandle *h);
+
 /* ------------------------------------------------ iterative path (additive) */
 
 /* Options of the GPU preconditioned Krylov solver. */

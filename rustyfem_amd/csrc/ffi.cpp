@@ -289,9 +289,226 @@ Handle* factor_mode(size_t n, size_t nn, const int64_t* loc, const double* coord
   return nullptr;
 }
 
+// Many independent problems as one forest: each problem keeps its own
+// analysis (its ordering is the one rustyfem_factor computes for it alone);
+// the trees are concatenated (equations, fronts and Lower entries offset), so
+// every launch of the level schedule covers the fronts of all problems.
+rf::Symbolic merge_symbolic(std::vector<std::unique_ptr<Handle>>& subs, std::vector<int64_t>& colptr,
+                            std::vector<int64_t>& rowidx, std::vector<int64_t>& eq_off, std::vector<int64_t>& low_off) {
+  rf::Symbolic S;
+  colptr.assign(1, 0);
+  rowidx.clear();
+  eq_off.assign(subs.size() + 1, 0);
+  low_off.assign(subs.size() + 1, 0);
+  int64_t e0 = 0, f0 = 0;
+  for (size_t i = 0; i < subs.size(); ++i) {
+    rf::Symbolic& A = *subs[i]->sym;
+    eq_off[i] = e0;
+    low_off[i] = static_cast<int64_t>(rowidx.size());
+    for (const rf::Front& F : A.fronts) {
+      rf::Front G = F;
+      G.start += e0;
+      for (auto& r : G.rows) r += e0;
+      for (auto& c : G.children) c += f0;
+      if (G.parent >= 0) G.parent += f0;
+      S.fronts.push_back(std::move(G));
+    }
+    for (int64_t r : A.roots) S.roots.push_back(r + f0);
+    const int64_t base = static_cast<int64_t>(rowidx.size());
+    for (int64_t c = 0; c < A.n; ++c) colptr.push_back(base + subs[i]->colptr[c + 1]);
+    for (int64_t r : subs[i]->rowidx) rowidx.push_back(r + e0);
+    e0 += A.n;
+    f0 += static_cast<int64_t>(A.fronts.size());
+  }
+  eq_off[subs.size()] = e0;
+  low_off[subs.size()] = static_cast<int64_t>(rowidx.size());
+  S.n = e0;
+  return S;
+}
+
+}  // namespace
+
+struct BatchHandle {
+  std::vector<std::unique_ptr<Handle>> subs;  // per problem: maps and entry filter (no device state)
+  std::vector<int64_t> n_off;                 // caller DOF offsets
+  std::vector<int64_t> eq_off, low_off;       // internal equation / Lower entry offsets
+  std::unique_ptr<rf::GpuFactor> gpu;
+  Options opt;
+  int64_t n_total = 0, nl = 0, nfronts = 0, nlevels = 0;
+  double work = 0.0;
+  mutable std::mutex solve_mu;
+};
+
+namespace {
+
+// numeric factorization of the batch; fills err (one per problem) and returns false on a failing pivot
+bool batch_numeric(BatchHandle* h, const double* const* data, int64_t* err) {
+  std::vector<double> v(h->nl, 0.0);
+  for (size_t i = 0; i < h->subs.size(); ++i) {
+    const Handle* s = h->subs[i].get();
+    for (size_t p = 0; p < s->vmap.size(); ++p)
+      if (s->vmap[p] >= 0) v[h->low_off[i] + s->vmap[p]] += data[i][p];
+  }
+  const int64_t e = h->gpu->factor(v.data(), false);
+  if (e < 0) return true;
+  const size_t i = static_cast<size_t>(std::upper_bound(h->eq_off.begin(), h->eq_off.end(), e) - h->eq_off.begin()) - 1;
+  if (err) err[i] = h->subs[i]->owner[e - h->eq_off[i]];
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
+
+BatchHandle* rustyfem_factor_batch(size_t nprob, const size_t* n, const size_t* nn, const int64_t* const* loc,
+                                   const double* const* coords, const int64_t* const* indptr,
+                                   const int64_t* const* indices, const double* const* data,
+                                   const rustyfem_options* opt, int64_t* err_dof) {
+  set_error("");
+  for (size_t i = 0; i < nprob; ++i) err_dof[i] = -1;
+  try {
+    auto h = std::make_unique<BatchHandle>();
+    h->opt = resolve(opt);
+    h->n_off.assign(nprob + 1, 0);
+    bool bad = false;
+    for (size_t i = 0; i < nprob; ++i) {
+      int64_t e = -1;
+      std::unique_ptr<Handle> s(build(n[i], nn[i], loc[i], coords[i], indptr[i], indices[i], data[i], Mode::Pd,
+                                      h->opt, &e, false));
+      if (!s) {
+        err_dof[i] = e;
+        bad = true;
+        continue;
+      }
+      h->n_off[i + 1] = h->n_off[i] + static_cast<int64_t>(n[i]);
+      h->work += s->work;
+      h->subs.push_back(std::move(s));
+    }
+    if (bad) return nullptr;
+    std::vector<int64_t> colptr, rowidx;
+    rf::Symbolic S = merge_symbolic(h->subs, colptr, rowidx, h->eq_off, h->low_off);
+    h->n_total = h->n_off[nprob];
+    h->nl = static_cast<int64_t>(rowidx.size());
+    h->nfronts = static_cast<int64_t>(S.fronts.size());
+    std::vector<int64_t> eq(h->n_total, -1);
+    std::vector<double> diag(h->n_total, 0.0);
+    for (size_t i = 0; i < nprob; ++i) {
+      const Handle* s = h->subs[i].get();
+      for (int64_t j = 0; j < s->n; ++j) {
+        eq[h->n_off[i] + j] = s->eq[j] >= 0 ? s->eq[j] + h->eq_off[i] : -1;
+        diag[h->n_off[i] + j] = s->diag[j];
+      }
+    }
+    rf::Plan plan = rf::build_plan(S, colptr, rowidx);
+    h->nlevels = static_cast<int64_t>(plan.levels.size());
+    h->gpu = rf::make_gpu_factor(std::move(plan), colptr, rowidx, h->opt.prec);
+    h->gpu->set_caller_map(eq, diag);
+    for (auto& s : h->subs) s->sym.reset();
+    if (!batch_numeric(h.get(), data, err_dof)) return nullptr;
+    return h.release();
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return nullptr;
+}
+
+int32_t rustyfem_refactor_batch(BatchHandle* h, const double* const* data, int64_t* err_dof) {
+  for (size_t i = 0; i < h->subs.size(); ++i) err_dof[i] = -1;
+  try {
+    if (!data) {
+      const int64_t e = h->gpu->factor(nullptr, true);
+      if (e < 0) return 0;
+      const size_t i = static_cast<size_t>(std::upper_bound(h->eq_off.begin(), h->eq_off.end(), e) - h->eq_off.begin()) - 1;
+      err_dof[i] = h->subs[i]->owner[e - h->eq_off[i]];
+      return 1;
+    }
+    return batch_numeric(h, data, err_dof) ? 0 : 1;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int32_t rustyfem_solve_batch(const BatchHandle* h, size_t nrhs, double* const* b) {
+  if (h->n_total == 0 || nrhs == 0) return 0;
+  try {
+    std::lock_guard<std::mutex> lock(h->solve_mu);
+    const int64_t N = h->n_total;
+    std::vector<double> all(static_cast<size_t>(N) * nrhs);
+    for (size_t i = 0; i < h->subs.size(); ++i) {
+      const int64_t ni = h->n_off[i + 1] - h->n_off[i];
+      for (size_t j = 0; j < nrhs; ++j)
+        std::memcpy(all.data() + j * N + h->n_off[i], b[i] + j * ni, ni * sizeof(double));
+    }
+    h->gpu->solve(all.data(), static_cast<int>(nrhs), false, h->opt.refine_tol, h->opt.max_refine);
+    for (size_t i = 0; i < h->subs.size(); ++i) {
+      const int64_t ni = h->n_off[i + 1] - h->n_off[i];
+      for (size_t j = 0; j < nrhs; ++j)
+        std::memcpy(b[i] + j * ni, all.data() + j * N + h->n_off[i], ni * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int32_t rustyfem_solve_batch_device(const BatchHandle* h, size_t nrhs, double* d_b) {
+  if (h->n_total == 0 || nrhs == 0) return 0;
+  try {
+    std::lock_guard<std::mutex> lock(h->solve_mu);
+    h->gpu->solve(d_b, static_cast<int>(nrhs), true, h->opt.refine_tol, h->opt.max_refine);
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_batch_info(const BatchHandle* h, double* out) {
+  // [problems, total DOFs, fronts, levels, work flops]
+  out[0] = static_cast<double>(h->subs.size());
+  out[1] = static_cast<double>(h->n_total);
+  out[2] = static_cast<double>(h->nfronts);
+  out[3] = static_cast<double>(h->nlevels);
+  out[4] = h->work;
+}
+
+void rustyfem_batch_stats(const BatchHandle* h, double* out, size_t len) {
+  double v[13] = {0};
+  const rf::Timings& t = h->gpu->timings();
+  v[0] = t.factor_ms;
+  v[1] = t.solve_ms;
+  v[2] = t.refine_ms;
+  v[3] = t.refine_steps;
+  v[4] = t.rel_residual;
+  v[5] = h->gpu->flops();
+  v[6] = static_cast<double>(h->gpu->device_bytes());
+  v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
+  v[8] = static_cast<double>(h->nfronts);
+  v[9] = static_cast<double>(h->nlevels);
+  v[10] = t.gemm_ms;
+  v[11] = t.gemm_launches;
+  v[12] = t.gemm_flops;
+  for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
+}
+
+void rustyfem_free_batch(BatchHandle* h) { delete h; }
+
+// order of problem i (as rustyfem_order of that problem alone)
+void rustyfem_batch_order(const BatchHandle* h, size_t i, int64_t* out) {
+  const Handle* s = h->subs[i].get();
+  int64_t next = s->n_sym;
+  for (int64_t j = 0; j < s->n; ++j) out[j] = s->eq[j] < 0 ? next++ : s->eq[j];
+}
 
 Handle* rustyfem_factor(size_t n, size_t nn, const int64_t* loc, const double* coords, const int64_t* indptr,
                         const int64_t* indices, const double* data, double* d_out, int64_t* err_dof) {

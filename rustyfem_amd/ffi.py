@@ -21,7 +21,9 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
            "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
            "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats", "rustyfem_pcg_setup",
-           "rustyfem_pcg_solve", "rustyfem_pcg_stats", "rustyfem_pcg_free", "rustyfem_solve_lower"]
+           "rustyfem_pcg_solve", "rustyfem_pcg_stats", "rustyfem_pcg_free", "rustyfem_solve_lower",
+           "rustyfem_factor_batch", "rustyfem_refactor_batch", "rustyfem_solve_batch", "rustyfem_solve_batch_device",
+           "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch"]
 
 _lib = None
 
@@ -89,6 +91,23 @@ def lib():
         L.rustyfem_pcg_stats.argtypes = [vp, vp, sz]
         L.rustyfem_pcg_free.restype = None
         L.rustyfem_pcg_free.argtypes = [vp]
+        pp = C.POINTER(C.c_void_p)
+        L.rustyfem_factor_batch.restype = vp
+        L.rustyfem_factor_batch.argtypes = [sz, vp, vp, pp, pp, pp, pp, pp, C.POINTER(Options), vp]
+        L.rustyfem_refactor_batch.restype = C.c_int32
+        L.rustyfem_refactor_batch.argtypes = [vp, pp, vp]
+        L.rustyfem_solve_batch.restype = C.c_int32
+        L.rustyfem_solve_batch.argtypes = [vp, sz, pp]
+        L.rustyfem_solve_batch_device.restype = C.c_int32
+        L.rustyfem_solve_batch_device.argtypes = [vp, sz, vp]
+        L.rustyfem_batch_info.restype = None
+        L.rustyfem_batch_info.argtypes = [vp, vp]
+        L.rustyfem_batch_stats.restype = None
+        L.rustyfem_batch_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_batch_order.restype = None
+        L.rustyfem_batch_order.argtypes = [vp, sz, vp]
+        L.rustyfem_free_batch.restype = None
+        L.rustyfem_free_batch.argtypes = [vp]
         L.rustyfem_solve_lower.restype = C.c_int64
         L.rustyfem_solve_lower.argtypes = [sz, vp, vp, vp, vp, C.c_int32, sz, vp, vp, C.c_double, C.c_int64, vp, f64p,
                                            C.POINTER(C.c_int32)]
@@ -299,4 +318,78 @@ class Pcg:
     def __del__(self):
         if getattr(self, "_h", None):
             lib().rustyfem_pcg_free(self._h)
+            self._h = None
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class Batch:
+    """Independent positive definite problems factored and solved together
+    (rustyfem_factor_batch)."""
+
+    def __init__(self, problems, precision=None, max_refine=-1, refine_tol=0.0):
+        self.problems = list(problems)
+        k = len(self.problems)
+        self.ns = np.array([int(p.n) for p in self.problems], dtype=np.uint64)
+        nns = np.array([int(p.nn) for p in self.problems], dtype=np.uint64)
+        self._keep = [_arrays(p)[0] for p in self.problems]
+        cols = list(zip(*self._keep))  # loc, coords, indptr, indices, data
+        self._ptr = [_ptrs(list(c)) for c in cols]
+        self._nns = nns
+        o = Options(1 if precision in (1, "mixed") else 0, 0, int(max_refine), 0, float(refine_tol))
+        self.err = np.full(max(k, 1), -1, dtype=np.int64)
+        self._h = lib().rustyfem_factor_batch(k, self.ns.ctypes.data, nns.ctypes.data, *self._ptr, C.byref(o),
+                                              self.err.ctypes.data)
+        self.err = self.err[:k]
+        self.off = np.concatenate([[0], np.cumsum(self.ns.astype(np.int64))])
+
+    @property
+    def ok(self):
+        return bool(self._h)
+
+    def refactor(self, datas=None):
+        err = np.full(len(self.problems), -1, dtype=np.int64)
+        if datas is None:
+            rc = lib().rustyfem_refactor_batch(self._h, None, err.ctypes.data)
+        else:
+            arrs = [np.ascontiguousarray(d, dtype=np.float64) for d in datas]
+            rc = lib().rustyfem_refactor_batch(self._h, _ptrs(arrs), err.ctypes.data)
+        if rc < 0:
+            raise RuntimeError(last_error())
+        return rc, err
+
+    def solve(self, rhs):
+        """rhs: list of (n_i, nrhs) arrays; returns the solutions."""
+        nrhs = rhs[0].shape[1] if rhs[0].ndim == 2 else 1
+        bs = [np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(len(b), -1)).copy(order="F") for b in rhs]
+        if lib().rustyfem_solve_batch(self._h, nrhs, _ptrs(bs)) != 0:
+            raise RuntimeError(last_error())
+        return bs
+
+    def solve_device(self, d_b, nrhs):
+        if lib().rustyfem_solve_batch_device(self._h, int(nrhs), d_b) != 0:
+            raise RuntimeError(last_error())
+
+    def info(self):
+        o = np.zeros(5)
+        lib().rustyfem_batch_info(self._h, o.ctypes.data)
+        return {"problems": int(o[0]), "n": int(o[1]), "fronts": int(o[2]), "levels": int(o[3]), "work": o[4]}
+
+    def stats(self):
+        o = np.zeros(13)
+        lib().rustyfem_batch_stats(self._h, o.ctypes.data, 13)
+        keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
+                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops"]
+        return dict(zip(keys, o.tolist()))
+
+    def order(self, i):
+        o = np.zeros(max(int(self.ns[i]), 1), dtype=np.int64)
+        lib().rustyfem_batch_order(self._h, i, o.ctypes.data)
+        return o[:int(self.ns[i])]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().rustyfem_free_batch(self._h)
             self._h = None
