@@ -29,6 +29,9 @@ constexpr int BC = 256;       // columns per backward tile (one per thread)
 constexpr int GT_M = 128;     // GEMM tile rows
 constexpr int EA_T = 64;      // extend-add tile edge
 constexpr int ZC = 64;        // columns per zero tile
+constexpr int SBW = 512;      // super-block width of the triangular solves
+constexpr int SRB = 64;       // rows per tile of the forward super-block update
+constexpr int PRT = 256;      // rows per tile of the backward super-block partial sums
 
 struct FrontDesc {
   int64_t off;     // element offset of the front in the arena
@@ -64,6 +67,16 @@ struct L21Tile { int32_t front, r0, c0, pad; };
 // sum of the L21 partials of columns [c0, c0 + BC) of a front: tiles [t0, t0 + nt) of the level
 struct L21Comb { int32_t front, c0, t0, nt; };
 
+// one super-block [J0, J1) of one front for the triangular solves: its 64-wide
+// pivot blocks are solved in sequence by one workgroup; bt is the index of the
+// first block's entry in Plan::btask.  Backward: the block's partial sums are
+// the np tiles from slot p0 of the step, nrc row tiles per 256-column chunk.
+struct SbTask { int32_t front, J0, J1, bt, p0, np, nrc, pad; };
+// forward: rows [r0, r0 + SRB) of the front receive -L[r, J0:J1] y[J0:J1]
+struct SbRowTile { int32_t front, J0, J1, r0; };
+// backward: partial sums over rows [r0, r0 + PRT) of L[r, c0:c1]^T t[r]
+struct SbPartTile { int32_t front, r0, c0, c1; };
+
 struct Range { int64_t begin = 0, count = 0; };
 
 enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel };
@@ -86,6 +99,9 @@ struct BwdPlan {
   std::vector<Range> steps;  // BwdTile ranges, step s handles block nb_f - 1 - s of every front
 };
 
+struct SbFwdStep { Range chain, rows; };
+struct SbBwdStep { Range part, chain; };
+
 struct LevelPlan {
   std::vector<int32_t> fronts;
   Range zero;      // ZeroTile
@@ -94,6 +110,8 @@ struct LevelPlan {
   std::vector<Phase> phases;       // factorization, in launch order
   std::vector<SolveStep> solve;    // inner blocks in pivot order (forward)
   BwdPlan bwd;                     // backward
+  std::vector<SbFwdStep> sfwd;     // super-block forward steps
+  std::vector<SbBwdStep> sbwd;     // super-block backward steps (last super-block first)
   Range fronts_r;  // range in the level-ordered front list (solve kernels)
 };
 
@@ -117,6 +135,12 @@ struct Plan {
   std::vector<L21Tile> l21;
   std::vector<L21Comb> l21c;
   std::vector<int32_t> bwd_last;  // DiagTask index of each front's last block, per level
+  std::vector<SbTask> sbt;        // super-block tasks (forward, then backward, per level)
+  std::vector<SbRowTile> sbrow;
+  std::vector<SbPartTile> sbpart;
+  std::vector<int32_t> btask;     // DiagTask of pivot block b of front f at btask[btask_off[f] + b]
+  std::vector<int32_t> btask_off;
+  int64_t max_sbpart = 0;         // most partial tiles in one backward step
   std::vector<int32_t> child_slot_fronts[2];  // fronts that are child slot s (per level ranges below)
   std::vector<Range> child_slot_range[2];     // per level
   int64_t arena_elems = 0;

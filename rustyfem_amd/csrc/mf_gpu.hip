@@ -819,6 +819,236 @@ __global__ __launch_bounds__(256) void k_bwd_finish(const FrontDesc* __restrict_
     for (int i = threadIdx.x; i < d.k; i += 256) x[j * n + d.start + i] = t[j * tv + d.toff + i];
 }
 
+// ------------------------------------------------ super-block triangular solves
+//
+// A front's pivot columns are taken SBW at a time.  Forward: one workgroup
+// solves the super-block's 64-wide blocks in sequence from LDS (V_b t_b, then
+// the super-block's rows below block b), then one launch over SRB-row tiles
+// applies the whole super-block to every row below it.  Backward: partial sums
+// of L[rows below, J]^T t over PRT-row tiles, then one workgroup per
+// super-block subtracts them in tile order and back-substitutes its blocks
+// from the last (V_b^T t_b, then the super-block's earlier columns).
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_sfwd_chain(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
+                                                    const T* __restrict__ arena, const double* __restrict__ vbuf,
+                                                    const int32_t* __restrict__ btask, double* __restrict__ t,
+                                                    int64_t tv, int nrhs) {
+  const SbTask tk = tasks[blockIdx.x];
+  const FrontDesc d = fd[tk.front];
+  __shared__ double tJ[SBW][R];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
+  const int W = tk.J1 - tk.J0, tid = threadIdx.x;
+  double* tf = t + d.toff + tk.J0;
+  const T* F = arena + d.off;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < W * R; idx += 256) {
+      const int i = idx % W, j = idx / W;
+      tJ[i][j] = j < nr ? tf[(j0 + j) * tv + i] : 0.0;
+    }
+    for (int b0 = 0; b0 < W; b0 += NB) {
+      const int jb = min(NB, W - b0);
+      __syncthreads();
+      load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB]) * NB * NB, Vs);
+      for (int idx = tid; idx < NB * R; idx += 256) {
+        const int i = idx % NB, j = idx / NB;
+        tmp[i][j] = i < jb ? tJ[b0 + i][j] : 0.0;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < NB * R; idx += 256) {
+        const int i = idx % NB, j = idx / NB;
+        if (i < jb) {
+          double s = 0.0;
+          for (int q = 0; q <= i; ++q) s = fma(Vs[q][i], tmp[q][j], s);
+          tJ[b0 + i][j] = s;
+        }
+      }
+      __syncthreads();
+      for (int r = b0 + jb + tid; r < W; r += 256) {
+        double acc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = 0.0;
+        const T* row = F + static_cast<int64_t>(tk.J0 + b0) * d.m + tk.J0 + r;
+#pragma unroll 8
+        for (int p = 0; p < jb; ++p) {
+          const double l = static_cast<double>(row[static_cast<int64_t>(p) * d.m]);
+#pragma unroll
+          for (int j = 0; j < R; ++j) acc[j] = fma(l, tJ[b0 + p][j], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) tJ[r][j] -= acc[j];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < W * R; idx += 256) {
+      const int i = idx % W, j = idx / W;
+      if (j < nr) tf[(j0 + j) * tv + i] = tJ[i][j];
+    }
+  }
+}
+
+// t[r] -= L[r, J0:J1] y[J0:J1] for SRB rows; the four waves split the columns
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__ fd, const SbRowTile* __restrict__ tiles,
+                                                   const T* __restrict__ arena, double* __restrict__ t, int64_t tv,
+                                                   int nrhs) {
+  const SbRowTile tl = tiles[blockIdx.x];
+  const FrontDesc d = fd[tl.front];
+  __shared__ double y[SBW][R];
+  __shared__ double red[4][SRB][R + 1];
+  const int W = tl.J1 - tl.J0, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = tl.r0 + lane;
+  const bool valid = r < d.m;
+  const int cw = (W + 3) / 4, cb = w * cw, ce = min(W, cb + cw);
+  double* tf = t + d.toff;
+  const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + r;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < W * R; idx += 256) {
+      const int i = idx % W, j = idx / W;
+      y[i][j] = j < nr ? tf[(j0 + j) * tv + tl.J0 + i] : 0.0;
+    }
+    __syncthreads();
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    if (valid) {
+#pragma unroll 8
+      for (int c = cb; c < ce; ++c) {
+        const double l = static_cast<double>(F[static_cast<int64_t>(c) * d.m]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(l, y[c][j], acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
+    __syncthreads();
+    if (tid < SRB && valid)
+      for (int j = 0; j < nr; ++j)
+        tf[(j0 + j) * tv + r] -= (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
+  }
+}
+
+// partial[tile][j][c] = sum over the tile's rows of L[r, c] t[r][j], c in [c0, c1)
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__ fd, const SbPartTile* __restrict__ tiles,
+                                                   const T* __restrict__ arena, const double* __restrict__ t,
+                                                   int64_t tv, int nrhs, double* __restrict__ partial) {
+  const SbPartTile tl = tiles[blockIdx.x];
+  const FrontDesc d = fd[tl.front];
+  constexpr int SL = 32;
+  __shared__ T Ls[BC][SL + 1];
+  __shared__ double ts[SL][R];
+  const int tid = threadIdx.x;
+  const int rend = min(tl.r0 + PRT, d.m);
+  const T* F = arena + d.off;
+  const double* tf = t + d.toff;
+  double* out = partial + static_cast<int64_t>(blockIdx.x) * nrhs * BC;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    for (int rs = tl.r0; rs < rend; rs += SL) {
+      __syncthreads();
+#pragma unroll 4
+      for (int e = tid; e < SL * BC; e += 256) {
+        const int rr = e % SL, cc = e / SL;
+        const int r = rs + rr, c = tl.c0 + cc;
+        Ls[cc][rr] = (r < rend && c < tl.c1) ? F[r + static_cast<int64_t>(c) * d.m] : T(0);
+      }
+      for (int e = tid; e < SL * R; e += 256) {
+        const int rr = e % SL, j = e / SL;
+        ts[rr][j] = (rs + rr < rend && j < nr) ? tf[(j0 + j) * tv + rs + rr] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int rr = 0; rr < SL; ++rr) {
+        const double l = static_cast<double>(Ls[tid][rr]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(l, ts[rr][j], acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j < nr) out[(j0 + j) * BC + tid] = acc[j];
+  }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
+                                                    const T* __restrict__ arena, const double* __restrict__ vbuf,
+                                                    const int32_t* __restrict__ btask, const double* __restrict__ partial,
+                                                    double* __restrict__ t, int64_t tv, int nrhs) {
+  const SbTask tk = tasks[blockIdx.x];
+  const FrontDesc d = fd[tk.front];
+  __shared__ double tJ[SBW][R];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double tmp[NB][R + 1];
+  const int W = tk.J1 - tk.J0, tid = threadIdx.x;
+  double* tf = t + d.toff + tk.J0;
+  const T* F = arena + d.off;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < W * R; idx += 256) {
+      const int i = idx % W, j = idx / W;
+      double v = 0.0;
+      if (j < nr) {
+        v = tf[(j0 + j) * tv + i];
+        const int cc = i / BC, cl = i % BC;
+        const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j0 + j) * BC + cl;
+        double s = 0.0;
+        for (int rc = 0; rc < tk.nrc; ++rc) s += src[static_cast<int64_t>(rc) * nrhs * BC];
+        v -= s;
+      }
+      tJ[i][j] = v;
+    }
+    for (int b0 = ((W - 1) / NB) * NB; b0 >= 0; b0 -= NB) {
+      const int jb = min(NB, W - b0);
+      __syncthreads();
+      load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB]) * NB * NB, Vs);
+      for (int idx = tid; idx < NB * R; idx += 256) {
+        const int i = idx % NB, j = idx / NB;
+        tmp[i][j] = i < jb ? tJ[b0 + i][j] : 0.0;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < NB * R; idx += 256) {
+        const int p = idx % NB, j = idx / NB;
+        if (p < jb) {
+          double s = 0.0;
+          for (int i = p; i < jb; ++i) s = fma(Vs[p][i], tmp[i][j], s);
+          tJ[b0 + p][j] = s;
+        }
+      }
+      __syncthreads();
+      for (int c = tid; c < b0; c += 256) {
+        double acc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = 0.0;
+        const T* col = F + static_cast<int64_t>(tk.J0 + c) * d.m + tk.J0 + b0;
+#pragma unroll 8
+        for (int i = 0; i < jb; ++i) {
+          const double l = static_cast<double>(col[i]);
+#pragma unroll
+          for (int j = 0; j < R; ++j) acc[j] = fma(l, tJ[b0 + i][j], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) tJ[c][j] -= acc[j];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < W * R; idx += 256) {
+      const int i = idx % W, j = idx / W;
+      if (j < nr) tf[(j0 + j) * tv + i] = tJ[i][j];
+    }
+  }
+}
+
 // ------------------------------------------------------------- refinement
 
 __global__ void k_gather(const int64_t* __restrict__ map, const double* __restrict__ src, double* __restrict__ dst,
@@ -941,6 +1171,14 @@ class Impl final : public GpuFactor {
     d_l21_ = dupload(P_.l21, bytes_, s_);
     d_l21c_ = dupload(P_.l21c, bytes_, s_);
     d_last_ = dupload(P_.bwd_last, bytes_, s_);
+    d_sbt_ = dupload(P_.sbt, bytes_, s_);
+    d_sbrow_ = dupload(P_.sbrow, bytes_, s_);
+    d_sbpart_ = dupload(P_.sbpart, bytes_, s_);
+    d_btask_ = dupload(P_.btask, bytes_, s_);
+    {
+      const char* e = std::getenv("RUSTYFEM_LEGACY_SOLVE");
+      legacy_solve_ = e && *e && *e != '0';
+    }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
     d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems), bytes_);
@@ -990,7 +1228,7 @@ class Impl final : public GpuFactor {
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
     drop_graphs();
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_};
     for (void* p : ptrs)
@@ -1148,7 +1386,8 @@ class Impl final : public GpuFactor {
     d_x_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_b_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_r_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
-    d_part_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.max_l21, 1)) * nrhs * BC, bytes_);
+    d_part_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(std::max(P_.max_l21, P_.max_sbpart), 1)) * nrhs * BC,
+                             bytes_);
     d_red_ = dalloc<double>(static_cast<size_t>(kRedBlocks) * nrhs, bytes_);
     d_nrm_ = dalloc<double>(nrhs, bytes_);
     d_bc_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(nc_, 1)) * nrhs, bytes_);
@@ -1235,6 +1474,10 @@ class Impl final : public GpuFactor {
 
   template <int R>
   void solve_levels(double* dx, int nrhs) {
+    if (!legacy_solve_) {
+      solve_levels_sb<R>(dx, nrhs);
+      return;
+    }
     const int64_t tv = P_.tvec_elems;
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
@@ -1262,6 +1505,38 @@ class Impl final : public GpuFactor {
       launch(k_bwd_last<R>, bp.last.count, 256, d_fd_, d_diag_, d_last_ + bp.last.begin, d_vbuf_, d_t_, tv, nrhs);
       for (const Range& st : bp.steps)
         launch(k_bwd_step<T, R>, st.count, 256, d_fd_, d_btile_ + st.begin, d_arena_, d_vbuf_, d_t_, tv, nrhs);
+      launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+    }
+  }
+
+  template <int R>
+  void solve_levels_sb(double* dx, int nrhs) {
+    const int64_t tv = P_.tvec_elems;
+    for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
+      const LevelPlan& lp = P_.levels[L];
+      const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      launch(k_fwd_init, lp.fronts_r.count, 256, d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      for (int sl = 0; sl < 2; ++sl) {
+        const Range& cr = P_.child_slot_range[sl][L];
+        launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
+      }
+      for (const SbFwdStep& st : lp.sfwd) {
+        launch(k_sfwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
+               d_t_, tv, nrhs);
+        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_t_, tv, nrhs);
+      }
+      launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
+    }
+    for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
+      const LevelPlan& lp = P_.levels[L];
+      const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      for (const SbBwdStep& st : lp.sbwd) {
+        launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
+               d_part_);
+        launch(k_sbwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
+               d_part_, d_t_, tv, nrhs);
+      }
       launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
     }
   }
@@ -1298,6 +1573,11 @@ class Impl final : public GpuFactor {
   BwdTile* d_btile_ = nullptr;
   L21Tile* d_l21_ = nullptr;
   L21Comb* d_l21c_ = nullptr;
+  SbTask* d_sbt_ = nullptr;
+  SbRowTile* d_sbrow_ = nullptr;
+  SbPartTile* d_sbpart_ = nullptr;
+  int32_t* d_btask_ = nullptr;
+  bool legacy_solve_ = false;
   int32_t* d_last_ = nullptr;
   int32_t* d_lf_ = nullptr;
   int32_t* d_cs_[2] = {nullptr, nullptr};

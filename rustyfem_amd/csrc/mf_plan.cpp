@@ -71,6 +71,15 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
         P.amap[p] = d.off + lr + (c - F.start) * static_cast<int64_t>(d.m);
       }
   }
+  P.btask_off.resize(P.nf);
+  {
+    int32_t o = 0;
+    for (int64_t f = 0; f < P.nf; ++f) {
+      P.btask_off[f] = o;
+      o += (P.fd[f].k + NB - 1) / NB;
+    }
+    P.btask.assign(o, -1);
+  }
   int32_t nlev = 0;
   for (int32_t l : P.level_of) nlev = std::max(nlev, l + 1);
   P.levels.resize(nlev);
@@ -138,6 +147,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
           for (int32_t q = 0; q < local; ++q) P.srow[sb + t0 + q].ntiles = local;
           if (last_tiles[f].first >= 0)  // the previous block's tiles compute this block's solve
             for (int32_t q = 0; q < last_tiles[f].second; ++q) P.srow[last_tiles[f].first + q].next = task;
+          P.btask[P.btask_off[f] + j0 / NB] = task;
           P.diag.push_back({f, j0, jb, t0, local, 0});
           last_tiles[f] = {sb + t0, local};
         }
@@ -208,6 +218,45 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
           }
         }
         bp.steps.push_back(take(tb0, static_cast<int64_t>(P.btile.size())));
+      }
+    }
+    // super-block solves of the level
+    {
+      int32_t maxnsb = 0;
+      for (int32_t f : lp.fronts) maxnsb = std::max(maxnsb, (P.fd[f].k + SBW - 1) / SBW);
+      for (int32_t st = 0; st < maxnsb; ++st) {
+        const int64_t cb = static_cast<int64_t>(P.sbt.size()), rb = static_cast<int64_t>(P.sbrow.size());
+        for (int32_t f : lp.fronts) {
+          const FrontDesc& d = P.fd[f];
+          const int32_t J0 = st * SBW;
+          if (J0 >= d.k) continue;
+          const int32_t J1 = std::min(J0 + SBW, d.k);
+          P.sbt.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, 0, 0, 0, 0});
+          for (int32_t r0 = J1; r0 < d.m; r0 += SRB) P.sbrow.push_back({f, J0, J1, r0});
+        }
+        lp.sfwd.push_back({take(cb, static_cast<int64_t>(P.sbt.size())), take(rb, static_cast<int64_t>(P.sbrow.size()))});
+      }
+      for (int32_t q = 0; q < maxnsb; ++q) {
+        const int64_t pb = static_cast<int64_t>(P.sbpart.size());
+        std::vector<SbTask> chain;
+        for (int32_t f : lp.fronts) {
+          const FrontDesc& d = P.fd[f];
+          const int32_t nsb = (d.k + SBW - 1) / SBW;
+          if (nsb <= q) continue;
+          const int32_t J0 = (nsb - 1 - q) * SBW, J1 = std::min(J0 + SBW, d.k);
+          const int32_t nrc = (d.m - J1 + PRT - 1) / PRT;
+          const int32_t p0 = static_cast<int32_t>(P.sbpart.size() - pb);
+          for (int32_t c0 = J0; c0 < J1; c0 += BC)
+            for (int32_t rc = 0; rc < nrc; ++rc)
+              P.sbpart.push_back({f, J1 + rc * PRT, c0, std::min(c0 + BC, J1)});
+          const int32_t np = static_cast<int32_t>(P.sbpart.size() - pb) - p0;
+          chain.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, p0, np, nrc, 0});
+        }
+        const int64_t cb = static_cast<int64_t>(P.sbt.size());
+        P.sbt.insert(P.sbt.end(), chain.begin(), chain.end());
+        const Range pr = take(pb, static_cast<int64_t>(P.sbpart.size()));
+        P.max_sbpart = std::max(P.max_sbpart, pr.count);
+        lp.sbwd.push_back({pr, take(cb, static_cast<int64_t>(P.sbt.size()))});
       }
     }
     // Schur update of the update matrices: lower tiles of rows/cols [k, m), K = [0, k)

@@ -52,7 +52,100 @@ std::string check_rects(std::vector<Rect>& v, const char* what) {
 
 }  // namespace
 
+// the super-block solve schedule: every index in bounds, every super-block of
+// every front solved once in order, row tiles covering the rows below exactly
+std::string check_solve_plan(const Plan& P) {
+  char buf[256];
+  for (int64_t f = 0; f < P.nf; ++f) {
+    const FrontDesc& d = P.fd[f];
+    for (int32_t b = 0; b < (d.k + NB - 1) / NB; ++b) {
+      const int64_t q = P.btask_off[f] + b;
+      if (q >= static_cast<int64_t>(P.btask.size()) || P.btask[q] < 0 ||
+          P.btask[q] >= static_cast<int32_t>(P.diag.size()) || P.diag[P.btask[q]].front != f ||
+          P.diag[P.btask[q]].j0 != b * NB) {
+        snprintf(buf, sizeof buf, "solve plan: block %d of front %lld has no pivot-block inverse", b, static_cast<long long>(f));
+        return buf;
+      }
+    }
+  }
+  for (size_t L = 0; L < P.levels.size(); ++L) {
+    const LevelPlan& lp = P.levels[L];
+    std::vector<int32_t> next(P.nf, 0);  // forward: next expected J0 per front
+    for (const SbFwdStep& st : lp.sfwd) {
+      for (int64_t q = st.chain.begin; q < st.chain.begin + st.chain.count; ++q) {
+        const SbTask& t = P.sbt[q];
+        const FrontDesc& d = P.fd[t.front];
+        if (t.J0 != next[t.front] || t.J1 <= t.J0 || t.J1 > d.k || t.J1 - t.J0 > SBW ||
+            t.bt != P.btask_off[t.front] + t.J0 / NB) {
+          snprintf(buf, sizeof buf, "solve plan: level %zu forward super-block [%d,%d) of front %d out of order or bounds", L,
+                   t.J0, t.J1, t.front);
+          return buf;
+        }
+        next[t.front] = t.J1;
+        int32_t rexp = t.J1;
+        for (int64_t r = st.rows.begin; r < st.rows.begin + st.rows.count; ++r) {
+          const SbRowTile& rt = P.sbrow[r];
+          if (rt.front != t.front) continue;
+          if (rt.J0 != t.J0 || rt.J1 != t.J1 || rt.r0 != rexp || rt.r0 >= d.m) {
+            snprintf(buf, sizeof buf, "solve plan: level %zu forward row tile r0=%d of front %d", L, rt.r0, rt.front);
+            return buf;
+          }
+          rexp += SRB;
+        }
+        if (rexp < d.m) {
+          snprintf(buf, sizeof buf, "solve plan: level %zu forward rows of front %d not covered", L, t.front);
+          return buf;
+        }
+      }
+    }
+    for (int32_t f : lp.fronts)
+      if (next[f] != P.fd[f].k) {
+        snprintf(buf, sizeof buf, "solve plan: level %zu forward super-blocks of front %d stop at %d of %d", L, f, next[f],
+                 P.fd[f].k);
+        return buf;
+      }
+    std::vector<int32_t> prev(P.nf);  // backward: expected J1 per front
+    for (int32_t f : lp.fronts) prev[f] = P.fd[f].k;
+    for (const SbBwdStep& st : lp.sbwd) {
+      for (int64_t q = st.chain.begin; q < st.chain.begin + st.chain.count; ++q) {
+        const SbTask& t = P.sbt[q];
+        const FrontDesc& d = P.fd[t.front];
+        if (t.J1 != prev[t.front] || t.J0 < 0 || t.J0 >= t.J1 || t.J1 - t.J0 > SBW || t.J0 % SBW != 0 ||
+            t.bt != P.btask_off[t.front] + t.J0 / NB) {
+          snprintf(buf, sizeof buf, "solve plan: level %zu backward super-block [%d,%d) of front %d", L, t.J0, t.J1, t.front);
+          return buf;
+        }
+        prev[t.front] = t.J0;
+        const int32_t ncc = (t.J1 - t.J0 + BC - 1) / BC, nrc = (d.m - t.J1 + PRT - 1) / PRT;
+        if (t.nrc != nrc || t.np != ncc * nrc || t.p0 < 0 || t.p0 + t.np > st.part.count) {
+          snprintf(buf, sizeof buf, "solve plan: level %zu backward partial range of front %d", L, t.front);
+          return buf;
+        }
+        for (int32_t cc = 0; cc < ncc; ++cc)
+          for (int32_t rc = 0; rc < nrc; ++rc) {
+            const SbPartTile& pt = P.sbpart[st.part.begin + t.p0 + cc * nrc + rc];
+            if (pt.front != t.front || pt.r0 != t.J1 + rc * PRT || pt.r0 >= d.m || pt.c0 != t.J0 + cc * BC ||
+                pt.c1 != std::min(pt.c0 + BC, t.J1)) {
+              snprintf(buf, sizeof buf, "solve plan: level %zu partial tile (%d,%d) of front %d", L, cc, rc, t.front);
+              return buf;
+            }
+          }
+      }
+    }
+    for (int32_t f : lp.fronts)
+      if (prev[f] != 0) {
+        snprintf(buf, sizeof buf, "solve plan: level %zu backward super-blocks of front %d stop at %d", L, f, prev[f]);
+        return buf;
+      }
+  }
+  return "";
+}
+
 std::string check_plan(const Plan& P, int plant) {
+  {
+    const std::string e = check_solve_plan(P);
+    if (!e.empty()) return e;
+  }
   for (size_t L = 0; L < P.levels.size(); ++L) {
     const LevelPlan& lp = P.levels[L];
     // extend-add: each launch writes parent entries (map[x], map[y]); they must be distinct

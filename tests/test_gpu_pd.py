@@ -145,3 +145,17 @@ def test_graph_replays_match_eager(rf, monkeypatch):
         assert he.refactor(None) == -1 and hg.refactor(None) == -1
         y = he.solve(p.rhs[:, :1])
         assert np.array_equal(hg.solve(p.rhs[:, :1]), y)
+
+
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_superblock_solve_matches_blockwise_solve(rf, monkeypatch, precision):
+    """The super-block triangular solves (512-column chains) against the
+    block-by-block schedule on the same factor: same solution to rounding."""
+    p = meshes.hex_block(9, nrhs=5, seed=3)
+    monkeypatch.setenv("RUSTYFEM_LEGACY_SOLVE", "1")
+    h_old, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+    monkeypatch.delenv("RUSTYFEM_LEGACY_SOLVE")
+    h_new, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+    x_old = h_old.solve(p.rhs)
+    x_new = h_new.solve(p.rhs)
+    assert np.max(np.abs(x_new - x_old)) / np.max(np.abs(x_old)) < 1e-12
