@@ -403,6 +403,7 @@ BatchHandle* rustyfem_factor_batch(size_t nprob, const size_t* n, const size_t* 
     h->nlevels = static_cast<int64_t>(plan.levels.size());
     h->gpu = rf::make_gpu_factor(std::move(plan), colptr, rowidx, h->opt.prec);
     h->gpu->set_caller_map(eq, diag);
+    h->gpu->set_segments(h->eq_off);
     for (auto& s : h->subs) s->sym.reset();
     if (!batch_numeric(h.get(), data, err_dof)) return nullptr;
     return h.release();

@@ -41,6 +41,10 @@ class GpuFactor {
   virtual void sync() = 0;
   // bracket every Schur-update GEMM launch with HIP events during factor()
   virtual void set_kernel_timing(bool on) = 0;
+  // Independent problems sharing the factor (a batch): internal equations
+  // [bounds[i], bounds[i+1]) belong to problem i, and refinement tests the
+  // residual of every problem and column separately.
+  virtual void set_segments(const std::vector<int64_t>& bounds) = 0;
 };
 
 // lower: Lower pattern (internal numbering), used to build the refinement matrix

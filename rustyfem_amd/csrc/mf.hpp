@@ -79,7 +79,10 @@ struct SbPartTile { int32_t front, r0, c0, c1; };
 
 struct Range { int64_t begin = 0, count = 0; };
 
-enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel };
+// Panel: the fused inner step (every tile factors the pivot block itself);
+// PanelDiag / PanelRows: the same step as two launches (pivot blocks, then
+// the row tiles below them)
+enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel, PanelDiag, PanelRows };
 
 struct Phase {
   PhaseKind kind;

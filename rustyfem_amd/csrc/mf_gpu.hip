@@ -400,6 +400,209 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
   }
 }
 
+// The inner pivot step split in two launches.  k_pdiag (one workgroup per
+// front): left-looking update of the 64x64 pivot block from the outer block's
+// columns [J0, j0), LDL^T with the pivot test and the inverse in the
+// registers of one wave; writes D and V = L_BB^-1.  k_prow (one workgroup per
+// 64-row tile below the block): left-looking update of the tile's rows, then
+// L_R = A_R L_BB^-T D^-1 on MFMA with W built from V and D.  The K chunks
+// are prefetched into registers while the previous chunk is multiplied.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
+                                               const T* __restrict__ arena, double* __restrict__ dpiv,
+                                               const double* __restrict__ adiag, double* __restrict__ vbuf,
+                                               unsigned long long* __restrict__ err) {
+  using M = Mfma<double>;
+  constexpr int KS = 32;
+  constexpr int NL = KS * NB / 256;
+  const PanelTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T* F = arena + d.off;
+  const int64_t m = d.m;
+  __shared__ double Ad[NB][NB + 1];
+  __shared__ double Vs[NB][NB + 1];
+  __shared__ double Ys[KS][NB + 2];
+  __shared__ double Y0[KS][NB + 2];
+  M::acc_t cb[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) cb[bj] = M::acc_t{0, 0, 0, 0};
+  double py[NL], pd[NL];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int e = tid + 256 * s, i = e % NB, k = k0 + e / NB;
+      const bool kin = k < t.j0;
+      py[s] = (kin && i < jb) ? static_cast<double>(F[t.j0 + i + k * m]) : 0.0;
+      pd[s] = kin ? dpiv[d.start + k] : 0.0;
+    }
+  };
+  if (t.J0 < t.j0) gload(t.J0);
+  for (int k0 = t.J0; k0 < t.j0; k0 += KS) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int e = tid + 256 * s, i = e % NB, kk = e / NB;
+      Y0[kk][i] = py[s];
+      Ys[kk][i] = py[s] * pd[s];
+    }
+    __syncthreads();
+    if (k0 + KS < t.j0) gload(k0 + KS);
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+      const int kr = kk + lane / 16;
+      const double b0 = Y0[kr][wave * 16 + lane % 16];
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) cb[bj] = M::op(Ys[kr][bj * 16 + lane % 16], b0, cb[bj]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      const int i = wave * 16 + lane % 16;
+      double fb = 0.0;
+      if (i < jb && q < jb) {
+        const int hi = i > q ? i : q, lo = i > q ? q : i;
+        fb = static_cast<double>(F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m]) - cb[bj][r];
+      }
+      Ad[q][i] = fb;
+    }
+  __syncthreads();
+  if (wave == 0) {
+    double a[NB], v[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) a[q] = Ad[q][lane];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < jb) {
+        const double dj = readlane_d(a[j], j);
+        if (lane == 0) {
+          const int64_t eq = static_cast<int64_t>(d.start) + t.j0 + j;
+          const double orig = adiag[eq];
+          if (!(orig > 0.0) || !(dj > 1e-12 * orig)) atomicMin(err, static_cast<unsigned long long>(eq));
+        }
+        const double l = a[j] / dj;
+#pragma unroll
+        for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, readlane_d(a[q], j), a[q]);
+        if (lane > j) a[j] = l;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) v[q] = q == lane ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      if (p < jb) {
+        const double lip = lane > p ? a[p] : 0.0;
+#pragma unroll
+        for (int q = 0; q <= p; ++q) v[q] = fma(-lip, readlane_d(v[q], p), v[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const bool in = lane < jb && q < jb;
+      Ad[q][lane] = in && q <= lane ? a[q] : 0.0;
+      Vs[q][lane] = in && q <= lane ? v[q] : 0.0;
+    }
+  }
+  __syncthreads();
+  double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+  for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
+  if (tid < jb) dpiv[d.start + t.j0 + tid] = Ad[tid][tid];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prow(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
+                                              T* __restrict__ arena, const double* __restrict__ dpiv,
+                                              const double* __restrict__ vbuf) {
+  using M = Mfma<double>;
+  constexpr int KS = 32;
+  constexpr int NL = KS * NB / 256;
+  const PanelTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  __shared__ double Ys[KS][NB + 2];   // L[j0 + q, k] d_k
+  __shared__ double Xs[KS][NB + 2];   // L[r0 + i, k]
+  __shared__ double Wd[NB][NB + 1];   // W[p][q] = L_BB^-1[q][p]
+  __shared__ double dinv[NB];
+  M::acc_t cr[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) cr[bj] = M::acc_t{0, 0, 0, 0};
+  double py[NL], px[NL];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int e = tid + 256 * s, i = e % NB, k = k0 + e / NB;
+      const bool kin = k < t.j0;
+      const double y = (kin && i < jb) ? static_cast<double>(F[t.j0 + i + k * m]) : 0.0;
+      py[s] = kin ? y * dpiv[d.start + k] : 0.0;
+      px[s] = (kin && t.r0 + i < d.m) ? static_cast<double>(F[t.r0 + i + k * m]) : 0.0;
+    }
+  };
+  if (t.J0 < t.j0) gload(t.J0);
+  for (int k0 = t.J0; k0 < t.j0; k0 += KS) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int e = tid + 256 * s, i = e % NB, kk = e / NB;
+      Ys[kk][i] = py[s];
+      Xs[kk][i] = px[s];
+    }
+    __syncthreads();
+    if (k0 + KS < t.j0) gload(k0 + KS);
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+      const int kr = kk + lane / 16;
+      const double bx = Xs[kr][wave * 16 + lane % 16];
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) cr[bj] = M::op(Ys[kr][bj * 16 + lane % 16], bx, cr[bj]);
+    }
+  }
+  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int q = idx % NB, p = idx / NB;
+    Wd[p][q] = q < jb ? vb[idx] : 0.0;
+  }
+  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dpiv[d.start + t.j0 + tid] : 0.0;
+  // A_R = F_R - C_R stays in the registers that hold C_R: at TRSM step s the
+  // lane's B operand is column 4s + lane/16 of its row, which is register
+  // (s / 4, s % 4) of cr
+  const int i = wave * 16 + lane % 16;
+  const int row = t.r0 + i;
+  double ar[16];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      ar[bj * 4 + r] = (row < d.m && q < jb) ? static_cast<double>(F[row + static_cast<int64_t>(t.j0 + q) * m]) - cr[bj][r]
+                                            : 0.0;
+    }
+  __syncthreads();
+  M::acc_t o[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s4 = 0; s4 < 16; ++s4) {
+    const int kr = 4 * s4 + lane / 16;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Wd[kr][bj * 16 + lane % 16], ar[s4], o[bj]);
+  }
+  if (row < d.m) {
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = bj * 16 + M::orow(lane, r);
+        if (q < jb) F[row + static_cast<int64_t>(t.j0 + q) * m] = T(o[bj][r] * dinv[q]);
+      }
+  }
+}
+
 // C[r0:r0+128, c0:min(c0+TN, cend)) -= (L[r0:, k0:k1] D) L[c0:, k0:k1]^T on MFMA.
 // Four waves tile 128 x TN (TN = 64: 4 x 1 waves of 32 x 64; TN = 128: 2 x 2
 // waves of 64 x 64).  The MFMA row index runs over C's columns so that lanes
@@ -1093,6 +1296,25 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict_
   }
 }
 
+// sums of squares of every segment [seg[s], seg[s+1]) of every column, one block per segment
+__global__ __launch_bounds__(256) void k_seg_sumsq(const double* __restrict__ v, int64_t n, int nrhs,
+                                                   const int64_t* __restrict__ seg, int nseg, double* __restrict__ out) {
+  __shared__ double sh[4];
+  for (int sg = blockIdx.x; sg < nseg; sg += gridDim.x)
+    for (int j = 0; j < nrhs; ++j) {
+      double s = 0.0;
+      for (int64_t i = seg[sg] + threadIdx.x; i < seg[sg + 1]; i += 256) {
+        const double x = v[j * n + i];
+        s += x * x;
+      }
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+      __syncthreads();
+      if (threadIdx.x == 0) out[static_cast<int64_t>(sg) * nrhs + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    }
+}
+
 __global__ void k_sumsq_final(const double* __restrict__ part, int nblk, int nrhs, double* __restrict__ out) {
   for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
     double s = 0.0;
@@ -1178,6 +1400,8 @@ class Impl final : public GpuFactor {
     {
       const char* e = std::getenv("RUSTYFEM_LEGACY_SOLVE");
       legacy_solve_ = e && *e && *e != '0';
+      const char* q = std::getenv("RUSTYFEM_LEGACY_PANEL");
+      legacy_panel_ = q && *q && *q != '0';
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -1230,7 +1454,7 @@ class Impl final : public GpuFactor {
     drop_graphs();
     void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
-                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_};
+                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
@@ -1269,6 +1493,21 @@ class Impl final : public GpuFactor {
           case PhaseKind::Panel:
             launch(k_panel<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
                    d_err_);
+            break;
+          case PhaseKind::PanelDiag:
+            if (legacy_panel_)
+              launch(k_panel<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
+                     d_err_);
+            else
+              launch(k_pdiag<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
+                     d_err_);
+            break;
+          case PhaseKind::PanelRows:
+            if (legacy_panel_)
+              launch(k_panel<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
+                     d_err_);
+            else
+              launch(k_prow<T>, ph.r.count, 256, d_fd_, d_panel_ + ph.r.begin, d_arena_, d_dpiv_, d_vbuf_);
             break;
         }
       }
@@ -1398,16 +1637,31 @@ class Impl final : public GpuFactor {
     const int64_t cnt = n_ * nrhs;
     HIPCHK(hipMemcpyAsync(d_x_, d_b_, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
     solve_device(d_x_, nrhs);
-    std::vector<double> bn(nrhs), rn(nrhs);
-    sumsq(d_b_, nrhs, bn.data());
+    const int nseg = nseg_ > 0 ? nseg_ : 1;
+    std::vector<double> bn(static_cast<size_t>(nseg) * nrhs), rn(static_cast<size_t>(nseg) * nrhs);
+    auto norms = [&](const double* v, double* out) {
+      if (nseg_ > 0) {
+        if (!d_segsq_ || segsq_cap_ < nseg * nrhs) {
+          if (d_segsq_) HIPCHK(hipFree(d_segsq_));
+          segsq_cap_ = nseg * nrhs;
+          d_segsq_ = dalloc<double>(segsq_cap_, bytes_);
+        }
+        launch(k_seg_sumsq, std::min(nseg, 4096), 256, v, n_, nrhs, d_seg_, nseg, d_segsq_);
+        HIPCHK(hipMemcpyAsync(out, d_segsq_, static_cast<size_t>(nseg) * nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+        HIPCHK(hipStreamSynchronize(s_));
+      } else {
+        sumsq(v, nrhs, out);
+      }
+    };
+    norms(d_b_, bn.data());
     int steps = 0;
     double worst = 0.0;
     for (;;) {
       launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, d_x_, d_b_, d_r_, n_, nrhs);
-      sumsq(d_r_, nrhs, rn.data());
+      norms(d_r_, rn.data());
       worst = 0.0;
       bool done = true;
-      for (int j = 0; j < nrhs; ++j) {
+      for (int j = 0; j < nseg * nrhs; ++j) {
         const double rel = bn[j] > 0 ? std::sqrt(rn[j] / bn[j]) : 0.0;
         worst = std::max(worst, rel);
         if (!(rel <= tol)) done = false;
@@ -1419,6 +1673,14 @@ class Impl final : public GpuFactor {
     }
     tm_.refine_steps = steps;
     tm_.rel_residual = worst;
+  }
+
+  void set_segments(const std::vector<int64_t>& bounds) override {
+    if (d_seg_) HIPCHK(hipFree(d_seg_));
+    d_seg_ = nullptr;
+    nseg_ = bounds.size() >= 2 ? static_cast<int>(bounds.size()) - 1 : 0;
+    if (nseg_ > 0) d_seg_ = dupload(bounds, bytes_, s_);
+    HIPCHK(hipStreamSynchronize(s_));
   }
 
   void sumsq(const double* v, int nrhs, double* host_out) {
@@ -1578,6 +1840,10 @@ class Impl final : public GpuFactor {
   SbPartTile* d_sbpart_ = nullptr;
   int32_t* d_btask_ = nullptr;
   bool legacy_solve_ = false;
+  int64_t* d_seg_ = nullptr;
+  double* d_segsq_ = nullptr;
+  int nseg_ = 0, segsq_cap_ = 0;
+  bool legacy_panel_ = false;
   int32_t* d_last_ = nullptr;
   int32_t* d_lf_ = nullptr;
   int32_t* d_cs_[2] = {nullptr, nullptr};

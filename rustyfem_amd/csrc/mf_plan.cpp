@@ -129,13 +129,20 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
         const int64_t db = static_cast<int64_t>(P.diag.size());
         const int64_t pb = static_cast<int64_t>(P.panel.size());
         const int64_t sb = static_cast<int64_t>(P.srow.size());
+        // pivot-block tiles of every front first, then the row tiles
+        for (int32_t f : lp.fronts) {
+          const FrontDesc& d = P.fd[f];
+          if (d.k <= j0) continue;
+          P.panel.push_back({f, J0, j0, std::min(NB, d.k - j0), j0, static_cast<int32_t>(P.diag.size()) +
+                             static_cast<int32_t>(P.panel.size() - pb)});
+        }
+        const int64_t prb = static_cast<int64_t>(P.panel.size());
         for (int32_t f : lp.fronts) {
           const FrontDesc& d = P.fd[f];
           if (d.k <= j0) continue;
           const int32_t jb = std::min(NB, d.k - j0);
           const int32_t j1 = j0 + jb;
           const int32_t task = static_cast<int32_t>(P.diag.size());
-          P.panel.push_back({f, J0, j0, jb, j0, task});
           for (int32_t r0 = j1; r0 < d.m; r0 += TRS) P.panel.push_back({f, J0, j0, jb, r0, task});
           const int32_t t0 = static_cast<int32_t>(P.srow.size() - sb);
           int32_t local = 0;
@@ -151,7 +158,8 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
           P.diag.push_back({f, j0, jb, t0, local, 0});
           last_tiles[f] = {sb + t0, local};
         }
-        phase(PhaseKind::Panel, pb, static_cast<int64_t>(P.panel.size()), 0);
+        phase(PhaseKind::PanelDiag, pb, prb, 0);
+        phase(PhaseKind::PanelRows, prb, static_cast<int64_t>(P.panel.size()), 0);
         lp.solve.push_back(SolveStep{take(db, static_cast<int64_t>(P.diag.size())),
                                      take(sb, static_cast<int64_t>(P.srow.size()))});
       }

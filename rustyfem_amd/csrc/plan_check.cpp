@@ -171,7 +171,25 @@ std::string check_plan(const Plan& P, int plant) {
     }
     for (const Phase& ph : lp.phases) {
       std::vector<Rect> v;
-      if (ph.kind == PhaseKind::Panel) {
+      if (ph.kind == PhaseKind::PanelDiag || ph.kind == PhaseKind::PanelRows) {
+        for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
+          const PanelTile& pt = P.panel[t];
+          const int32_t j1 = pt.j0 + pt.jb, f = pt.front, m = P.fd[f].m, id = static_cast<int32_t>(t);
+          const int32_t rend = std::min(pt.r0 + TRS, m);
+          if ((ph.kind == PhaseKind::PanelDiag) != (pt.r0 == pt.j0)) return "level " + std::to_string(L) + " panel tile kind";
+          v.push_back({f, pt.j0, j1, pt.J0, pt.j0, false, id});  // Y
+          if (ph.kind == PhaseKind::PanelDiag) {
+            v.push_back({f, pt.j0, j1, pt.j0, j1, false, id});   // the pivot block (only read)
+          } else {
+            // planted: the first row tile of a front writes the updated rows of
+            // the pivot block back over Y, which the other row tiles read
+            if (plant == 1 && (t == ph.r.begin || P.panel[t - 1].front != f))
+              v.push_back({f, pt.j0, j1, pt.J0, pt.j0, true, id});
+            v.push_back({f, pt.r0, rend, pt.J0, pt.j0, false, id});  // X
+            v.push_back({f, pt.r0, rend, pt.j0, j1, true, id});      // A_R -> L_R
+          }
+        }
+      } else if (ph.kind == PhaseKind::Panel) {
         for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
           const PanelTile& pt = P.panel[t];
           const int32_t j1 = pt.j0 + pt.jb, f = pt.front, m = P.fd[f].m, id = static_cast<int32_t>(t);
@@ -198,7 +216,8 @@ std::string check_plan(const Plan& P, int plant) {
       } else {
         continue;
       }
-      const std::string e = check_rects(v, ph.kind == PhaseKind::Panel ? "panel" : "gemm");
+      const bool panel = ph.kind == PhaseKind::Panel || ph.kind == PhaseKind::PanelDiag || ph.kind == PhaseKind::PanelRows;
+      const std::string e = check_rects(v, panel ? "panel" : "gemm");
       if (!e.empty()) return "level " + std::to_string(L) + " " + e;
     }
   }
