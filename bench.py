@@ -215,7 +215,7 @@ def run_batch(args):
         worst = max(worst, float(np.max(np.linalg.norm(p.rhs - p.to_scipy() @ xi, axis=0) /
                                         np.linalg.norm(p.rhs, axis=0))))
     t = float(np.median(times))
-    out = {"metric": "batched factor + 8-RHS solve of 512 plates to ||r||/||b|| <= 1e-10 (FP64, refined)",
+    out = {"metric": "batched factor + 8-RHS solve of 512 plates, FP64 refined per plate to 1e-10 or the FP64 floor",
            "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
            "config": {"workload": f"{len(plates)} CPE4 plates, nx~U{{20..200}}, ny=nx/10, 8 load cases, seed 3; "
@@ -233,6 +233,13 @@ def run_batch(args):
             h.solve(p.rhs)
             wsum += h.info()[0]
         dt = time.perf_counter() - t1
+        worst_o = 0.0
+        for p in sample:
+            h, _, _ = oracle.Handle.factor(p)
+            xo = h.solve(p.rhs)
+            worst_o = max(worst_o, float(np.max(np.linalg.norm(p.rhs - p.to_scipy() @ xo, axis=0) /
+                                                np.linalg.norm(p.rhs, axis=0))))
+        out["oracle_rel_residual_max_sample"] = worst_o
         out["cpu_baseline"] = {"value": dt * info["work"] / wsum, "unit": "s", "cores": int(oracle.threads()),
                                "kind": "port",
                                "sample": f"C oracle factor + 8-RHS solve of the first {len(sample)} plates "

@@ -32,7 +32,8 @@ class GpuFactor {
   // In-place solve of nrhs columns in caller numbering (column-major
   // n_caller x nrhs, host or device pointer).  With max_steps > 0 the solve
   // is followed by FP64 iterative refinement against the FP64 matrix until
-  // ||b - A x|| <= tol ||b|| for every column or max_steps corrections.
+  // ||b - A x|| <= tol ||b|| for every column, max_steps corrections, or a
+  // correction that does not halve the worst residual (FP64 floor reached).
   virtual void solve(double* b, int nrhs, bool on_device, double tol, int max_steps) = 0;
   virtual void pivots(double* d_host) = 0;
   virtual const Timings& timings() const = 0;

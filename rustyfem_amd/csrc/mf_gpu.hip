@@ -473,23 +473,28 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
   __syncthreads();
   if (wave == 0) {
     double a[NB], v[NB];
+    // the original diagonal of this lane's pivot, for the test after the elimination
+    const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
 #pragma unroll
     for (int q = 0; q < NB; ++q) a[q] = Ad[q][lane];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < jb) {
         const double dj = readlane_d(a[j], j);
-        if (lane == 0) {
-          const int64_t eq = static_cast<int64_t>(d.start) + t.j0 + j;
-          const double orig = adiag[eq];
-          if (!(orig > 0.0) || !(dj > 1e-12 * orig)) atomicMin(err, static_cast<unsigned long long>(eq));
-        }
         const double l = a[j] / dj;
 #pragma unroll
         for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, readlane_d(a[q], j), a[q]);
         if (lane > j) a[j] = l;
       }
     }
+    // pivot test (src/sparse.rs:528-531): lane j holds D_j in a[j]; the first
+    // failing equation wins through the minimum
+    double dl = 0.0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q == lane) dl = a[q];
+    if (lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
+      atomicMin(err, static_cast<unsigned long long>(static_cast<int64_t>(d.start) + t.j0 + lane));
 #pragma unroll
     for (int q = 0; q < NB; ++q) v[q] = q == lane ? 1.0 : 0.0;
 #pragma unroll
@@ -1655,7 +1660,7 @@ class Impl final : public GpuFactor {
     };
     norms(d_b_, bn.data());
     int steps = 0;
-    double worst = 0.0;
+    double worst = 0.0, prev = HUGE_VAL;
     for (;;) {
       launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, d_x_, d_b_, d_r_, n_, nrhs);
       norms(d_r_, rn.data());
@@ -1666,7 +1671,10 @@ class Impl final : public GpuFactor {
         worst = std::max(worst, rel);
         if (!(rel <= tol)) done = false;
       }
-      if (done || steps >= max_steps) break;
+      // stop at the target, at the step limit, or when a correction no longer
+      // halves the worst residual (the FP64 rounding floor of the residual)
+      if (done || steps >= max_steps || !(worst < 0.5 * prev)) break;
+      prev = worst;
       solve_device(d_r_, nrhs);
       launch(k_axpy, (cnt + 255) / 256, 256, d_x_, d_r_, cnt);
       ++steps;

@@ -159,3 +159,16 @@ def test_superblock_solve_matches_blockwise_solve(rf, monkeypatch, precision):
     x_old = h_old.solve(p.rhs)
     x_new = h_new.solve(p.rhs)
     assert np.max(np.abs(x_new - x_old)) / np.max(np.abs(x_old)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_split_panel_matches_fused_panel(rf, monkeypatch, precision):
+    """The inner pivot step as two launches (pivot blocks, then row tiles) does
+    the fused kernel's arithmetic: identical D and identical solutions."""
+    p = meshes.hex_block(10, nrhs=2, seed=4)
+    monkeypatch.setenv("RUSTYFEM_LEGACY_PANEL", "1")
+    h_old, _, d_old = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+    monkeypatch.delenv("RUSTYFEM_LEGACY_PANEL")
+    h_new, _, d_new = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+    assert np.array_equal(d_new, d_old)
+    assert np.array_equal(h_new.solve(p.rhs), h_old.solve(p.rhs))
