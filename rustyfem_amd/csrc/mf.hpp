@@ -31,7 +31,7 @@ constexpr int EA_T = 64;      // extend-add tile edge
 constexpr int ZC = 64;        // columns per zero tile
 constexpr int SBW = 512;      // super-block width of the triangular solves
 constexpr int SRB = 64;       // rows per tile of the forward super-block update
-constexpr int PRT = 256;      // rows per tile of the backward super-block partial sums
+constexpr int PRT = 64;       // rows per tile of the backward super-block partial sums
 
 struct FrontDesc {
   int64_t off;     // element offset of the front in the arena

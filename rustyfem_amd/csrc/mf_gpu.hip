@@ -537,6 +537,13 @@ __global__ __launch_bounds__(256) void k_prow(const FrontDesc* __restrict__ fd, 
   M::acc_t cr[4];
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj) cr[bj] = M::acc_t{0, 0, 0, 0};
+  // the pivot block's inverse and D (written by the previous launch) are
+  // requested first, so that their latency hides behind the update loop
+  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+  double wv[NB * NB / 256];
+#pragma unroll
+  for (int s = 0; s < NB * NB / 256; ++s) wv[s] = vb[tid + 256 * s];
+  const double dv = tid < jb ? dpiv[d.start + t.j0 + tid] : 1.0;
   double py[NL], px[NL];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -567,12 +574,12 @@ __global__ __launch_bounds__(256) void k_prow(const FrontDesc* __restrict__ fd, 
       for (int bj = 0; bj < 4; ++bj) cr[bj] = M::op(Ys[kr][bj * 16 + lane % 16], bx, cr[bj]);
     }
   }
-  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
-  for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int q = idx % NB, p = idx / NB;
-    Wd[p][q] = q < jb ? vb[idx] : 0.0;
+#pragma unroll
+  for (int s = 0; s < NB * NB / 256; ++s) {
+    const int idx = tid + 256 * s, q = idx % NB, p = idx / NB;
+    Wd[p][q] = q < jb ? wv[s] : 0.0;
   }
-  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dpiv[d.start + t.j0 + tid] : 0.0;
+  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dv : 0.0;
   // A_R = F_R - C_R stays in the registers that hold C_R: at TRSM step s the
   // lane's B operand is column 4s + lane/16 of its row, which is register
   // (s / 4, s % 4) of cr
@@ -1045,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_sfwd_chain(const FrontDesc* __restrict_
   const SbTask tk = tasks[blockIdx.x];
   const FrontDesc d = fd[tk.front];
   __shared__ double tJ[SBW][R];
-  __shared__ double Vs[NB][NB + 1];
+  __shared__ double Vb[2][NB][NB + 1];
   __shared__ double tmp[NB][R + 1];
   const int W = tk.J1 - tk.J0, tid = threadIdx.x;
   double* tf = t + d.toff + tk.J0;
@@ -1057,10 +1064,13 @@ __global__ __launch_bounds__(256) void k_sfwd_chain(const FrontDesc* __restrict_
       const int i = idx % W, j = idx / W;
       tJ[i][j] = j < nr ? tf[(j0 + j) * tv + i] : 0.0;
     }
+    load_v(vbuf + static_cast<int64_t>(btask[tk.bt]) * NB * NB, Vb[0]);
     for (int b0 = 0; b0 < W; b0 += NB) {
       const int jb = min(NB, W - b0);
+      double (*Vs)[NB + 1] = Vb[(b0 / NB) & 1];
       __syncthreads();
-      load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB]) * NB * NB, Vs);
+      // the next block's inverse streams into the other buffer meanwhile
+      if (b0 + NB < W) load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB + 1]) * NB * NB, Vb[((b0 / NB) + 1) & 1]);
       for (int idx = tid; idx < NB * R; idx += 256) {
         const int i = idx % NB, j = idx / NB;
         tmp[i][j] = i < jb ? tJ[b0 + i][j] : 0.0;
@@ -1195,14 +1205,16 @@ __global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict_
   const SbTask tk = tasks[blockIdx.x];
   const FrontDesc d = fd[tk.front];
   __shared__ double tJ[SBW][R];
-  __shared__ double Vs[NB][NB + 1];
+  __shared__ double Vb[2][NB][NB + 1];
   __shared__ double tmp[NB][R + 1];
   const int W = tk.J1 - tk.J0, tid = threadIdx.x;
   double* tf = t + d.toff + tk.J0;
   const T* F = arena + d.off;
+  const int last = ((W - 1) / NB) * NB;
   for (int j0 = 0; j0 < nrhs; j0 += R) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
+    load_v(vbuf + static_cast<int64_t>(btask[tk.bt + last / NB]) * NB * NB, Vb[(last / NB) & 1]);
     for (int idx = tid; idx < W * R; idx += 256) {
       const int i = idx % W, j = idx / W;
       double v = 0.0;
@@ -1216,10 +1228,11 @@ __global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict_
       }
       tJ[i][j] = v;
     }
-    for (int b0 = ((W - 1) / NB) * NB; b0 >= 0; b0 -= NB) {
+    for (int b0 = last; b0 >= 0; b0 -= NB) {
       const int jb = min(NB, W - b0);
+      double (*Vs)[NB + 1] = Vb[(b0 / NB) & 1];
       __syncthreads();
-      load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB]) * NB * NB, Vs);
+      if (b0 > 0) load_v(vbuf + static_cast<int64_t>(btask[tk.bt + b0 / NB - 1]) * NB * NB, Vb[((b0 / NB) - 1) & 1]);
       for (int idx = tid; idx < NB * R; idx += 256) {
         const int i = idx % NB, j = idx / NB;
         tmp[i][j] = i < jb ? tJ[b0 + i][j] : 0.0;
