@@ -71,7 +71,10 @@ struct L21Comb { int32_t front, c0, t0, nt; };
 // pivot blocks are solved in sequence by one workgroup; bt is the index of the
 // first block's entry in Plan::btask.  Backward: the block's partial sums are
 // the np tiles from slot p0 of the step, nrc row tiles per 256-column chunk.
-struct SbTask { int32_t front, J0, J1, bt, p0, np, nrc, pad; };
+struct SbTask { int32_t front, J0, J1, bt, p0, np, nrc, sb; };
+// explicit inverse of a super-block's unit-lower factor: block column b (64
+// wide) of X = L_JJ^-1, W x W column-major at Plan::sbinv_off[sb]
+struct SbInvTask { int32_t front, J0, W, b, bt, sb; };
 // forward: rows [r0, r0 + SRB) of the front receive -L[r, J0:J1] y[J0:J1]
 struct SbRowTile { int32_t front, J0, J1, r0; };
 // backward: partial sums over rows [r0, r0 + PRT) of L[r, c0:c1]^T t[r]
@@ -144,6 +147,10 @@ struct Plan {
   std::vector<int32_t> btask;     // DiagTask of pivot block b of front f at btask[btask_off[f] + b]
   std::vector<int32_t> btask_off;
   int64_t max_sbpart = 0;         // most partial tiles in one backward step
+  std::vector<SbInvTask> sbinv;   // all block columns of all super-block inverses
+  std::vector<int64_t> sbinv_off; // element offset of super-block sb's inverse
+  std::vector<int32_t> sb_first;  // index of front f's first super-block
+  int64_t sbinv_elems = 0;
   std::vector<int32_t> child_slot_fronts[2];  // fronts that are child slot s (per level ranges below)
   std::vector<Range> child_slot_range[2];     // per level
   int64_t arena_elems = 0;

@@ -1111,8 +1111,8 @@ __global__ __launch_bounds__(256) void k_sfwd_chain(const FrontDesc* __restrict_
 // t[r] -= L[r, J0:J1] y[J0:J1] for SRB rows; the four waves split the columns
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__ fd, const SbRowTile* __restrict__ tiles,
-                                                   const T* __restrict__ arena, double* __restrict__ t, int64_t tv,
-                                                   int nrhs) {
+                                                   const T* __restrict__ arena, const double* __restrict__ ysrc,
+                                                   double* __restrict__ t, int64_t tv, int nrhs) {
   const SbRowTile tl = tiles[blockIdx.x];
   const FrontDesc d = fd[tl.front];
   __shared__ double y[SBW][R];
@@ -1122,13 +1122,14 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
   const bool valid = r < d.m;
   const int cw = (W + 3) / 4, cb = w * cw, ce = min(W, cb + cw);
   double* tf = t + d.toff;
+  const double* yf = ysrc + d.toff;
   const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + r;
   for (int j0 = 0; j0 < nrhs; j0 += R) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
     for (int idx = tid; idx < W * R; idx += 256) {
       const int i = idx % W, j = idx / W;
-      y[i][j] = j < nr ? tf[(j0 + j) * tv + tl.J0 + i] : 0.0;
+      y[i][j] = j < nr ? yf[(j0 + j) * tv + tl.J0 + i] : 0.0;
     }
     __syncthreads();
     double acc[R];
@@ -1268,6 +1269,226 @@ __global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict_
       if (j < nr) tf[(j0 + j) * tv + i] = tJ[i][j];
     }
   }
+}
+
+// ---------------------------------------- super-block inverses (explicit)
+//
+// X = L_JJ^-1 for every super-block, built once per factorization from the
+// pivot-block inverses V_a and the off-diagonal 64x64 blocks of L:
+// X_bb = V_b, X_ab = -V_a sum_{c=b}^{a-1} L_ac X_cb (a > b), one workgroup
+// per 64-wide block column, FP64 MFMA.  The solves then apply X (forward) and
+// X^T (backward) as parallel products instead of a sequence of 64-wide blocks.
+
+// out[i][q] += sum_k Bt[i][k] Ak[k][q] for a 64x64 output (the k_prow layout:
+// lane row i = wave*16 + lane%16, column q = bj*16 + orow(lane, r))
+__device__ __forceinline__ void mm64(const double (*Ak)[NB + 1], const double (*Bt)[NB + 1], Mfma<double>::acc_t* o) {
+  using M = Mfma<double>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    const int kr = kk + lane / 16;
+    const double b = Bt[wave * 16 + lane % 16][kr];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Ak[kr][bj * 16 + lane % 16], b, o[bj]);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd, const SbInvTask* __restrict__ tasks,
+                                               const T* __restrict__ arena, const double* __restrict__ vbuf,
+                                               const int32_t* __restrict__ btask, const int64_t* __restrict__ sboff,
+                                               double* __restrict__ xinv) {
+  using M = Mfma<double>;
+  const SbInvTask v = tasks[blockIdx.x];
+  const FrontDesc d = fd[v.front];
+  __shared__ double As[NB][NB + 1];   // k-major operand (X_cb, then acc)
+  __shared__ double Bs[NB][NB + 1];   // row-major operand (L_ac, then V_a)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int W = v.W, b = v.b, jb = min(NB, W - b);
+  double* X = xinv + sboff[v.sb];
+  const T* F = arena + d.off;
+  const int64_t m = d.m;
+  // X_bb = V_b; the rest of the block column above the diagonal is zero
+  {
+    const double* vb = vbuf + static_cast<int64_t>(btask[v.bt]) * NB * NB;  // column-major V
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+      const int i = idx % NB, j = idx / NB;
+      if (i < jb && j < jb) X[(b + i) + static_cast<int64_t>(b + j) * W] = vb[idx];
+    }
+    for (int idx = tid; idx < b * jb; idx += 256) {
+      const int i = idx % b, j = idx / b;
+      X[i + static_cast<int64_t>(b + j) * W] = 0.0;
+    }
+  }
+  for (int a = b + NB; a < W; a += NB) {
+    const int ja = min(NB, W - a);
+    M::acc_t o[4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
+    for (int c = b; c < a; c += NB) {
+      const int jc = min(NB, W - c);
+      __syncthreads();
+      for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int i = idx % NB, j = idx / NB;
+        // As[p][j] = X[c + p][b + j]; Bs[i][p] = L[J0 + a + i][J0 + c + p]
+        As[i][j] = (i < jc && j < jb) ? X[(c + i) + static_cast<int64_t>(b + j) * W] : 0.0;
+        Bs[i][j] = (i < ja && j < jc) ? static_cast<double>(F[(v.J0 + a + i) + (v.J0 + c + j) * m]) : 0.0;
+      }
+      __syncthreads();
+      mm64(As, Bs, o);
+    }
+    // acc -> As (k-major: As[p][j] = acc[p][j]); V_a -> Bs (row-major Bs[i][p] = V_a[i][p])
+    __syncthreads();
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) As[wave * 16 + lane % 16][bj * 16 + M::orow(lane, r)] = o[bj][r];
+    {
+      const double* va = vbuf + static_cast<int64_t>(btask[v.bt + (a - b) / NB]) * NB * NB;
+      for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int i = idx % NB, p = idx / NB;
+        Bs[i][p] = va[idx];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
+    mm64(As, Bs, o);
+    const int i = wave * 16 + lane % 16;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = bj * 16 + M::orow(lane, r);
+        if (i < ja && q < jb) X[(a + i) + static_cast<int64_t>(b + q) * W] = -o[bj][r];
+      }
+    // the next step reads this block back from X
+    __syncthreads();
+  }
+}
+
+// forward: y_J = X t_J for rows [64a, 64a + 64) of the super-block (a = blockIdx.y);
+// the result goes to ys (t_J is read by the other row blocks)
+template <int R>
+__global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
+                                                  const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
+                                                  const double* __restrict__ t, double* __restrict__ ys, int64_t tv,
+                                                  int nrhs) {
+  const SbTask tk = tasks[blockIdx.x];
+  const int W = tk.J1 - tk.J0, a0 = blockIdx.y * NB;
+  if (a0 >= W) return;
+  const FrontDesc d = fd[tk.front];
+  __shared__ double tJ[SBW][R];
+  __shared__ double red[4][NB][R + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = a0 + lane;
+  const int jend = min(W, a0 + NB);          // X is lower triangular
+  const int cw = (jend + 3) / 4, cb = w * cw, ce = min(jend, cb + cw);
+  const double* X = xinv + sboff[tk.sb];
+  const double* tf = t + d.toff + tk.J0;
+  double* yf = ys + d.toff + tk.J0;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < jend * R; idx += 256) {
+      const int q = idx % jend, j = idx / jend;
+      tJ[q][j] = j < nr ? tf[(j0 + j) * tv + q] : 0.0;
+    }
+    __syncthreads();
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    if (i < W) {
+      const double* xr = X + i;
+#pragma unroll 8
+      for (int q = cb; q < ce; ++q) {
+        const double x = xr[static_cast<int64_t>(q) * W];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(x, tJ[q][j], acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
+    __syncthreads();
+    if (tid < NB && i < W)
+      for (int j = 0; j < nr; ++j)
+        yf[(j0 + j) * tv + i] = (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
+  }
+}
+
+// backward: z_J = t_J - (partial sums in tile order) -> ys
+template <int R>
+__global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
+                                                   const double* __restrict__ partial, const double* __restrict__ t,
+                                                   double* __restrict__ ys, int64_t tv, int nrhs) {
+  const SbTask tk = tasks[blockIdx.x];
+  const FrontDesc d = fd[tk.front];
+  const int W = tk.J1 - tk.J0;
+  const double* tf = t + d.toff + tk.J0;
+  double* zf = ys + d.toff + tk.J0;
+  for (int idx = threadIdx.x; idx < W * nrhs; idx += 256) {
+    const int i = idx % W, j = idx / W;
+    const int cc = i / BC, cl = i % BC;
+    const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j) * BC + cl;
+    double s = 0.0;
+    for (int rc = 0; rc < tk.nrc; ++rc) s += src[static_cast<int64_t>(rc) * nrhs * BC];
+    zf[j * tv + i] = tf[j * tv + i] - s;
+  }
+}
+
+// backward: x_J = X^T z_J for rows [64a, 64a + 64) (a = blockIdx.y), one wave per
+// output row at a time, lanes along the (contiguous) column of X
+template <int R>
+__global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
+                                                  const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
+                                                  const double* __restrict__ ys, double* __restrict__ t, int64_t tv,
+                                                  int nrhs) {
+  const SbTask tk = tasks[blockIdx.x];
+  const int W = tk.J1 - tk.J0, a0 = blockIdx.y * NB;
+  if (a0 >= W) return;
+  const FrontDesc d = fd[tk.front];
+  __shared__ double zs[SBW][R];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const double* X = xinv + sboff[tk.sb];
+  const double* zf = ys + d.toff + tk.J0;
+  double* tf = t + d.toff + tk.J0;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < (W - a0) * R; idx += 256) {
+      const int q = a0 + idx % (W - a0), j = idx / (W - a0);
+      zs[q][j] = j < nr ? zf[(j0 + j) * tv + q] : 0.0;
+    }
+    __syncthreads();
+    for (int ii = w; ii < NB; ii += 4) {
+      const int i = a0 + ii;
+      if (i >= W) break;
+      const double* xc = X + static_cast<int64_t>(i) * W;
+      double acc[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) acc[j] = 0.0;
+      for (int q = i + lane; q < W; q += 64) {
+        const double x = xc[q];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(x, zs[q][j], acc[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        for (int o = 32; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
+      if (lane == 0)
+        for (int j = 0; j < nr; ++j) tf[(j0 + j) * tv + i] = acc[j];
+    }
+  }
+}
+
+// x[pivots] = y / D from the forward result in ys
+__global__ __launch_bounds__(256) void k_fwd_finish_ys(const FrontDesc* __restrict__ fd, const int32_t* __restrict__ fronts,
+                                                       const double* __restrict__ ys, const double* __restrict__ dpiv,
+                                                       double* __restrict__ x, int64_t n, int64_t tv, int nrhs) {
+  const FrontDesc d = fd[fronts[blockIdx.x]];
+  for (int j = 0; j < nrhs; ++j)
+    for (int i = threadIdx.x; i < d.k; i += 256)
+      x[j * n + d.start + i] = ys[j * tv + d.toff + i] / dpiv[d.start + i];
 }
 
 // ------------------------------------------------------------- refinement
@@ -1415,11 +1636,16 @@ class Impl final : public GpuFactor {
     d_sbrow_ = dupload(P_.sbrow, bytes_, s_);
     d_sbpart_ = dupload(P_.sbpart, bytes_, s_);
     d_btask_ = dupload(P_.btask, bytes_, s_);
+    d_sbinv_ = dupload(P_.sbinv, bytes_, s_);
+    d_sboff_ = dupload(P_.sbinv_off, bytes_, s_);
+    d_xinv_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.sbinv_elems, 1)), bytes_);
     {
       const char* e = std::getenv("RUSTYFEM_LEGACY_SOLVE");
       legacy_solve_ = e && *e && *e != '0';
       const char* q = std::getenv("RUSTYFEM_LEGACY_PANEL");
       legacy_panel_ = q && *q && *q != '0';
+      const char* c = std::getenv("RUSTYFEM_SOLVE_CHAIN");
+      chain_solve_ = c && *c && *c != '0';
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -1470,7 +1696,7 @@ class Impl final : public GpuFactor {
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
     drop_graphs();
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
@@ -1530,6 +1756,10 @@ class Impl final : public GpuFactor {
         }
       }
     }
+    // explicit inverses of the super-blocks for the solves
+    if (!legacy_solve_ && !chain_solve_)
+      launch(k_sbinv<T>, static_cast<int64_t>(P_.sbinv.size()), 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
+             d_sboff_, d_xinv_);
     HIPCHK(hipEventRecord(e1_, s_));
     unsigned long long err = 0;
     HIPCHK(hipMemcpyAsync(&err, d_err_, sizeof err, hipMemcpyDeviceToHost, s_));
@@ -1640,6 +1870,8 @@ class Impl final : public GpuFactor {
       if (p) HIPCHK(hipFree(p));
     cap_rhs_ = nrhs;
     d_t_ = dalloc<double>(static_cast<size_t>(P_.tvec_elems) * nrhs, bytes_);
+    if (d_ys_) HIPCHK(hipFree(d_ys_));
+    d_ys_ = dalloc<double>(static_cast<size_t>(P_.tvec_elems) * nrhs, bytes_);
     d_x_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_b_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_r_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
@@ -1793,7 +2025,51 @@ class Impl final : public GpuFactor {
   }
 
   template <int R>
+  void solve_levels_inv(double* dx, int nrhs) {
+    const int64_t tv = P_.tvec_elems;
+    const int nbk = SBW / NB;
+    for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
+      const LevelPlan& lp = P_.levels[L];
+      const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      launch(k_fwd_init, lp.fronts_r.count, 256, d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      for (int sl = 0; sl < 2; ++sl) {
+        const Range& cr = P_.child_slot_range[sl][L];
+        launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
+      }
+      for (const SbFwdStep& st : lp.sfwd) {
+        if (st.chain.count > 0) {
+          hipLaunchKernelGGL(k_sfwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
+                             d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_t_, d_ys_, tv, nrhs);
+          HIPCHK(hipGetLastError());
+        }
+        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_ys_, d_t_, tv, nrhs);
+      }
+      launch(k_fwd_finish_ys, lp.fronts_r.count, 256, d_fd_, fr, d_ys_, d_dpiv_, dx, n_, tv, nrhs);
+    }
+    for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
+      const LevelPlan& lp = P_.levels[L];
+      const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      for (const SbBwdStep& st : lp.sbwd) {
+        launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
+               d_part_);
+        launch(k_sbwd_comb<R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
+        if (st.chain.count > 0) {
+          hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
+                             d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_ys_, d_t_, tv, nrhs);
+          HIPCHK(hipGetLastError());
+        }
+      }
+      launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+    }
+  }
+
+  template <int R>
   void solve_levels_sb(double* dx, int nrhs) {
+    if (!chain_solve_) {
+      solve_levels_inv<R>(dx, nrhs);
+      return;
+    }
     const int64_t tv = P_.tvec_elems;
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
@@ -1806,7 +2082,7 @@ class Impl final : public GpuFactor {
       for (const SbFwdStep& st : lp.sfwd) {
         launch(k_sfwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
                d_t_, tv, nrhs);
-        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_t_, tv, nrhs);
+        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_t_, d_t_, tv, nrhs);
       }
       launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
     }
@@ -1865,6 +2141,11 @@ class Impl final : public GpuFactor {
   double* d_segsq_ = nullptr;
   int nseg_ = 0, segsq_cap_ = 0;
   bool legacy_panel_ = false;
+  bool chain_solve_ = false;
+  SbInvTask* d_sbinv_ = nullptr;
+  int64_t* d_sboff_ = nullptr;
+  double* d_xinv_ = nullptr;
+  double* d_ys_ = nullptr;
   int32_t* d_last_ = nullptr;
   int32_t* d_lf_ = nullptr;
   int32_t* d_cs_[2] = {nullptr, nullptr};

@@ -80,6 +80,24 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     }
     P.btask.assign(o, -1);
   }
+  // super-block inverses: front f's super-blocks are sb_first[f] .. + nsb
+  P.sb_first.resize(P.nf);
+  {
+    int32_t sb = 0;
+    int64_t off = 0;
+    for (int64_t f = 0; f < P.nf; ++f) {
+      P.sb_first[f] = sb;
+      const int32_t k = P.fd[f].k;
+      for (int32_t J0 = 0; J0 < k; J0 += SBW, ++sb) {
+        const int32_t W = std::min(SBW, k - J0);
+        P.sbinv_off.push_back(off);
+        off += static_cast<int64_t>(W) * W;
+        for (int32_t b = 0; b < W; b += NB)
+          P.sbinv.push_back({static_cast<int32_t>(f), J0, W, b, P.btask_off[f] + (J0 + b) / NB, sb});
+      }
+    }
+    P.sbinv_elems = off;
+  }
   int32_t nlev = 0;
   for (int32_t l : P.level_of) nlev = std::max(nlev, l + 1);
   P.levels.resize(nlev);
@@ -239,7 +257,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
           const int32_t J0 = st * SBW;
           if (J0 >= d.k) continue;
           const int32_t J1 = std::min(J0 + SBW, d.k);
-          P.sbt.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, 0, 0, 0, 0});
+          P.sbt.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, 0, 0, 0, P.sb_first[f] + st});
           for (int32_t r0 = J1; r0 < d.m; r0 += SRB) P.sbrow.push_back({f, J0, J1, r0});
         }
         lp.sfwd.push_back({take(cb, static_cast<int64_t>(P.sbt.size())), take(rb, static_cast<int64_t>(P.sbrow.size()))});
@@ -258,7 +276,7 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
             for (int32_t rc = 0; rc < nrc; ++rc)
               P.sbpart.push_back({f, J1 + rc * PRT, c0, std::min(c0 + BC, J1)});
           const int32_t np = static_cast<int32_t>(P.sbpart.size() - pb) - p0;
-          chain.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, p0, np, nrc, 0});
+          chain.push_back({f, J0, J1, P.btask_off[f] + J0 / NB, p0, np, nrc, P.sb_first[f] + nsb - 1 - q});
         }
         const int64_t cb = static_cast<int64_t>(P.sbt.size());
         P.sbt.insert(P.sbt.end(), chain.begin(), chain.end());

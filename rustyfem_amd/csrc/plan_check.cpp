@@ -68,6 +68,16 @@ std::string check_solve_plan(const Plan& P) {
       }
     }
   }
+  for (const SbInvTask& v : P.sbinv) {
+    const FrontDesc& d = P.fd[v.front];
+    if (v.sb < 0 || v.sb >= static_cast<int32_t>(P.sbinv_off.size()) || v.sb != P.sb_first[v.front] + v.J0 / SBW ||
+        v.J0 + v.W > d.k || v.W > SBW || v.b >= v.W || v.b % NB != 0 ||
+        v.bt != P.btask_off[v.front] + (v.J0 + v.b) / NB ||
+        P.sbinv_off[v.sb] + static_cast<int64_t>(v.W) * v.W > P.sbinv_elems) {
+      snprintf(buf, sizeof buf, "solve plan: super-block inverse task of front %d [%d,+%d) column %d", v.front, v.J0, v.W, v.b);
+      return buf;
+    }
+  }
   for (size_t L = 0; L < P.levels.size(); ++L) {
     const LevelPlan& lp = P.levels[L];
     std::vector<int32_t> next(P.nf, 0);  // forward: next expected J0 per front
@@ -76,7 +86,7 @@ std::string check_solve_plan(const Plan& P) {
         const SbTask& t = P.sbt[q];
         const FrontDesc& d = P.fd[t.front];
         if (t.J0 != next[t.front] || t.J1 <= t.J0 || t.J1 > d.k || t.J1 - t.J0 > SBW ||
-            t.bt != P.btask_off[t.front] + t.J0 / NB) {
+            t.bt != P.btask_off[t.front] + t.J0 / NB || t.sb != P.sb_first[t.front] + t.J0 / SBW) {
           snprintf(buf, sizeof buf, "solve plan: level %zu forward super-block [%d,%d) of front %d out of order or bounds", L,
                    t.J0, t.J1, t.front);
           return buf;
@@ -111,7 +121,7 @@ std::string check_solve_plan(const Plan& P) {
         const SbTask& t = P.sbt[q];
         const FrontDesc& d = P.fd[t.front];
         if (t.J1 != prev[t.front] || t.J0 < 0 || t.J0 >= t.J1 || t.J1 - t.J0 > SBW || t.J0 % SBW != 0 ||
-            t.bt != P.btask_off[t.front] + t.J0 / NB) {
+            t.bt != P.btask_off[t.front] + t.J0 / NB || t.sb != P.sb_first[t.front] + t.J0 / SBW) {
           snprintf(buf, sizeof buf, "solve plan: level %zu backward super-block [%d,%d) of front %d", L, t.J0, t.J1, t.front);
           return buf;
         }

@@ -172,3 +172,18 @@ def test_split_panel_matches_fused_panel(rf, monkeypatch, precision):
     h_new, _, d_new = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
     assert np.array_equal(d_new, d_old)
     assert np.array_equal(h_new.solve(p.rhs), h_old.solve(p.rhs))
+
+
+def test_inverse_and_chain_superblock_solves_agree(rf, monkeypatch):
+    """Super-block solves with the explicit FP64 inverse X = L_JJ^-1 against the
+    in-LDS block chain on the same factor (a front with several super-blocks)."""
+    p = meshes.hex_block(14, nrhs=3, seed=8)
+    monkeypatch.setenv("RUSTYFEM_SOLVE_CHAIN", "1")
+    h_c, _, _ = rf.Handle.factor(p)
+    monkeypatch.delenv("RUSTYFEM_SOLVE_CHAIN")
+    h_i, _, _ = rf.Handle.factor(p)
+    x_c = h_c.solve(p.rhs)
+    x_i = h_i.solve(p.rhs)
+    assert np.max(np.abs(x_i - x_c)) / np.max(np.abs(x_c)) < 1e-11
+    a = p.to_scipy()
+    assert np.all(np.linalg.norm(p.rhs - a @ x_i, axis=0) / np.linalg.norm(p.rhs, axis=0) < 1e-12)
