@@ -92,6 +92,19 @@ def cpu_baseline(size, work_full, nrhs):
                        f"{rel:.1e}; value = full-workload flops / sample rate (+ solve scaled by nnz(L))")}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the Schur-update GEMM from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), if present."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_gemm_float_hex64.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def lower_of(p):
     import numpy as np
     import scipy.sparse as sp
@@ -298,7 +311,7 @@ def main():
     gemm_ms, gemm_n, gemm_flops = st["gemm_ms"], st["gemm_launches"], st["gemm_flops"]
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
-            "frac": (achieved / PEAK["fp32_mfma_tflops"]) if achieved else None, "traffic": None,
+            "frac": (achieved / PEAK["fp32_mfma_tflops"]) if achieved else None, "traffic": measured_traffic(),
             "kernel": "k_gemm<float> (Schur update, v_mfma_f32_16x16x4f32)",
             "launches": gemm_n, "avg_launch_ms": gemm_ms / gemm_n if gemm_n else None,
             "algorithmic_flops_per_launch": gemm_flops / gemm_n if gemm_n else None}
