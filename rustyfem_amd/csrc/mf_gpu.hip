@@ -1297,7 +1297,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd, const SbInvTask* __restrict__ tasks,
                                                const T* __restrict__ arena, const double* __restrict__ vbuf,
                                                const int32_t* __restrict__ btask, const int64_t* __restrict__ sboff,
-                                               double* __restrict__ xinv) {
+                                               double* __restrict__ xinv, double* __restrict__ xinvt) {
   using M = Mfma<double>;
   const SbInvTask v = tasks[blockIdx.x];
   const FrontDesc d = fd[v.front];
@@ -1306,6 +1306,7 @@ __global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int W = v.W, b = v.b, jb = min(NB, W - b);
   double* X = xinv + sboff[v.sb];
+  double* Xt = xinvt + sboff[v.sb];   // X^T, column-major (the backward reads its columns' rows)
   const T* F = arena + d.off;
   const int64_t m = d.m;
   // X_bb = V_b; the rest of the block column above the diagonal is zero
@@ -1313,7 +1314,11 @@ __global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd,
     const double* vb = vbuf + static_cast<int64_t>(btask[v.bt]) * NB * NB;  // column-major V
     for (int idx = tid; idx < NB * NB; idx += 256) {
       const int i = idx % NB, j = idx / NB;
-      if (i < jb && j < jb) X[(b + i) + static_cast<int64_t>(b + j) * W] = vb[idx];
+      if (i < jb && j < jb) {
+        const double x = vb[idx];
+        X[(b + i) + static_cast<int64_t>(b + j) * W] = x;
+        Xt[(b + j) + static_cast<int64_t>(b + i) * W] = x;
+      }
     }
     for (int idx = tid; idx < b * jb; idx += 256) {
       const int i = idx % b, j = idx / b;
@@ -1360,7 +1365,10 @@ __global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = bj * 16 + M::orow(lane, r);
-        if (i < ja && q < jb) X[(a + i) + static_cast<int64_t>(b + q) * W] = -o[bj][r];
+        if (i < ja && q < jb) {
+          X[(a + i) + static_cast<int64_t>(b + q) * W] = -o[bj][r];
+          Xt[(b + q) + static_cast<int64_t>(a + i) * W] = -o[bj][r];
+        }
       }
     // the next step reads this block back from X
     __syncthreads();
@@ -1416,31 +1424,51 @@ __global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ 
   }
 }
 
-// backward: z_J = t_J - (partial sums in tile order) -> ys
+// backward: z_J = t_J - (partial sums in row-tile order) -> ys; 64 columns per
+// block (blockIdx.y), the four waves sum contiguous quarters of the row tiles
 template <int R>
 __global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
                                                    const double* __restrict__ partial, const double* __restrict__ t,
                                                    double* __restrict__ ys, int64_t tv, int nrhs) {
   const SbTask tk = tasks[blockIdx.x];
+  const int W = tk.J1 - tk.J0, c0 = blockIdx.y * NB;
+  if (c0 >= W) return;
   const FrontDesc d = fd[tk.front];
-  const int W = tk.J1 - tk.J0;
+  __shared__ double red[4][NB][R + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = c0 + lane;
+  const int cc = i / BC, cl = i % BC;
+  const int qw = (tk.nrc + 3) / 4, rb = w * qw, re = min(tk.nrc, rb + qw);
   const double* tf = t + d.toff + tk.J0;
   double* zf = ys + d.toff + tk.J0;
-  for (int idx = threadIdx.x; idx < W * nrhs; idx += 256) {
-    const int i = idx % W, j = idx / W;
-    const int cc = i / BC, cl = i % BC;
-    const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j) * BC + cl;
-    double s = 0.0;
-    for (int rc = 0; rc < tk.nrc; ++rc) s += src[static_cast<int64_t>(rc) * nrhs * BC];
-    zf[j * tv + i] = tf[j * tv + i] - s;
+  for (int j0 = 0; j0 < nrhs; j0 += R) {
+    const int nr = min(R, nrhs - j0);
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    if (i < W) {
+      const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j0) * BC + cl;
+      for (int rc = rb; rc < re; ++rc)
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (j < nr) acc[j] += src[(static_cast<int64_t>(rc) * nrhs + j) * BC];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
+    __syncthreads();
+    if (tid < NB && i < W)
+      for (int j = 0; j < nr; ++j)
+        zf[(j0 + j) * tv + i] =
+            tf[(j0 + j) * tv + i] - ((red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]));
   }
 }
 
-// backward: x_J = X^T z_J for rows [64a, 64a + 64) (a = blockIdx.y), one wave per
-// output row at a time, lanes along the (contiguous) column of X
+// backward: x_J = X^T z_J for rows [64a, 64a + 64) (a = blockIdx.y) from the
+// stored transpose: lane = output row, the four waves split the columns
 template <int R>
 __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
-                                                  const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
+                                                  const double* __restrict__ xinvt, const int64_t* __restrict__ sboff,
                                                   const double* __restrict__ ys, double* __restrict__ t, int64_t tv,
                                                   int nrhs) {
   const SbTask tk = tasks[blockIdx.x];
@@ -1448,36 +1476,40 @@ __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ 
   if (a0 >= W) return;
   const FrontDesc d = fd[tk.front];
   __shared__ double zs[SBW][R];
+  __shared__ double red[4][NB][R + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const double* X = xinv + sboff[tk.sb];
+  const int i = a0 + lane;
+  const int span = W - a0;                      // X^T is upper triangular: columns q >= a0
+  const int cw = (span + 3) / 4, cb = a0 + w * cw, ce = min(W, cb + cw);
+  const double* Xt = xinvt + sboff[tk.sb];
   const double* zf = ys + d.toff + tk.J0;
   double* tf = t + d.toff + tk.J0;
   for (int j0 = 0; j0 < nrhs; j0 += R) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
-    for (int idx = tid; idx < (W - a0) * R; idx += 256) {
-      const int q = a0 + idx % (W - a0), j = idx / (W - a0);
+    for (int idx = tid; idx < span * R; idx += 256) {
+      const int q = a0 + idx % span, j = idx / span;
       zs[q][j] = j < nr ? zf[(j0 + j) * tv + q] : 0.0;
     }
     __syncthreads();
-    for (int ii = w; ii < NB; ii += 4) {
-      const int i = a0 + ii;
-      if (i >= W) break;
-      const double* xc = X + static_cast<int64_t>(i) * W;
-      double acc[R];
+    double acc[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) acc[j] = 0.0;
-      for (int q = i + lane; q < W; q += 64) {
-        const double x = xc[q];
+    for (int j = 0; j < R; ++j) acc[j] = 0.0;
+    if (i < W) {
+      const double* xr = Xt + i;
+#pragma unroll 8
+      for (int q = cb; q < ce; ++q) {
+        const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
         for (int j = 0; j < R; ++j) acc[j] = fma(x, zs[q][j], acc[j]);
       }
-#pragma unroll
-      for (int j = 0; j < R; ++j)
-        for (int o = 32; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
-      if (lane == 0)
-        for (int j = 0; j < nr; ++j) tf[(j0 + j) * tv + i] = acc[j];
     }
+#pragma unroll
+    for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
+    __syncthreads();
+    if (tid < NB && i < W)
+      for (int j = 0; j < nr; ++j)
+        tf[(j0 + j) * tv + i] = (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
   }
 }
 
@@ -1639,6 +1671,7 @@ class Impl final : public GpuFactor {
     d_sbinv_ = dupload(P_.sbinv, bytes_, s_);
     d_sboff_ = dupload(P_.sbinv_off, bytes_, s_);
     d_xinv_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.sbinv_elems, 1)), bytes_);
+    d_xinvt_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.sbinv_elems, 1)), bytes_);
     {
       const char* e = std::getenv("RUSTYFEM_LEGACY_SOLVE");
       legacy_solve_ = e && *e && *e != '0';
@@ -1696,7 +1729,7 @@ class Impl final : public GpuFactor {
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
     drop_graphs();
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_ys_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
@@ -1759,7 +1792,7 @@ class Impl final : public GpuFactor {
     // explicit inverses of the super-blocks for the solves
     if (!legacy_solve_ && !chain_solve_)
       launch(k_sbinv<T>, static_cast<int64_t>(P_.sbinv.size()), 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
-             d_sboff_, d_xinv_);
+             d_sboff_, d_xinv_, d_xinvt_);
     HIPCHK(hipEventRecord(e1_, s_));
     unsigned long long err = 0;
     HIPCHK(hipMemcpyAsync(&err, d_err_, sizeof err, hipMemcpyDeviceToHost, s_));
@@ -2053,10 +2086,12 @@ class Impl final : public GpuFactor {
       for (const SbBwdStep& st : lp.sbwd) {
         launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
                d_part_);
-        launch(k_sbwd_comb<R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
         if (st.chain.count > 0) {
+          hipLaunchKernelGGL(k_sbwd_comb<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
+                             d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
+          HIPCHK(hipGetLastError());
           hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
-                             d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_ys_, d_t_, tv, nrhs);
+                             d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, d_ys_, d_t_, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
       }
@@ -2145,6 +2180,7 @@ class Impl final : public GpuFactor {
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;
   double* d_xinv_ = nullptr;
+  double* d_xinvt_ = nullptr;
   double* d_ys_ = nullptr;
   int32_t* d_last_ = nullptr;
   int32_t* d_lf_ = nullptr;
