@@ -1,0 +1,136 @@
+// Schur-update GEMM of the multifrontal factorization on MFMA (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mf.hpp"
+#include "mfma.hpp"
+
+namespace rf {
+
+// C[r0:r0+128, c0:min(c0+128, cend)) -= (L[r0:, k0:k1] D[k0:k1]) L[c0:, k0:k1]^T
+// on one 128 x 128 tile of a column-major front.
+//
+// Four waves, 2 x 2, each own a 64 x 64 block (4 x 4 MFMA 16 x 16 outputs).
+// The MFMA row index runs over C's columns, so that lanes 0..15 of an output
+// register hold 16 consecutive rows of the front (coalesced C access).
+//
+// K tiles of KT columns (32 in FP32, 16 in FP64) are staged in LDS with k
+// contiguous per row: a lane's MFMA steps read their A, B and D operands with
+// 16-byte LDS reads (four steps per read in FP32, two in FP64; lane group
+// g = lane / 16 uses k = 16 h + 4 g + s at step s of chunk h, for both
+// operands).  Row
+// pitch KT + 16 bytes keeps the 16-lane reads on distinct banks.
+//
+// Loads carry no per-element masks, so that the next K tile stays in flight
+// behind the current tile's products: rows beyond the front (A) or beyond
+// cend (B) are clamped to the last valid row, whose finite lower-triangle
+// values reach only C entries that are not stored; columns beyond k1 are
+// clamped to k1 - 1 and their D is 0.  Column addresses are wave-uniform.
+// The C tile is read 16 entries at a time before it is written, so that the
+// reads of a group overlap.  Two waves per SIMD at least (the accumulators
+// then live in VGPRs; FP32 reaches three).
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm(
+    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles, T* __restrict__ arena,
+    const double* __restrict__ dpiv) {
+  using M = Mfma<T>;
+  using V4 = typename M::acc_t;
+  using L16 = typename M::lds16_t;
+  constexpr int E = 16 / static_cast<int>(sizeof(T));  // elements per 16-byte LDS read
+  constexpr int KT = M::KT;
+  constexpr int KP = KT + 16 / static_cast<int>(sizeof(T));
+  constexpr int NQ = KT / 8;
+  const GemmTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  __shared__ __attribute__((aligned(16))) T As[GT_M][KP];
+  __shared__ __attribute__((aligned(16))) T Bs[GT_M][KP];
+  __shared__ __attribute__((aligned(16))) T Ds[KT];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1, kg = wave >> 1;
+  const int li = tid & 127;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  const int arow = min(t.r0 + li, d.m - 1);
+  const int brow = min(t.c0 + li, t.cend - 1);
+  V4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = V4{0, 0, 0, 0};
+  T ra[4 * NQ], rb[4 * NQ];
+  double rd = 0.0;
+  // thread (li, kg) loads row li of both tiles at k = 4 (kg + 2 q) + j
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = min(kb + 4 * (kg + 2 * q) + j, t.k1 - 1);
+        const T* col = F + static_cast<int64_t>(c) * m;
+        ra[4 * q + j] = col[arow];
+        rb[4 * q + j] = col[brow];
+      }
+    if (tid < KT) rd = dpiv[d.start + min(kb + tid, t.k1 - 1)];
+  };
+  auto store = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int k4 = 4 * (kg + 2 * q);
+      lds_put4(&As[li][k4], ra[4 * q], ra[4 * q + 1], ra[4 * q + 2], ra[4 * q + 3]);
+      lds_put4(&Bs[li][k4], rb[4 * q], rb[4 * q + 1], rb[4 * q + 2], rb[4 * q + 3]);
+    }
+    if (tid < KT) Ds[tid] = kb + tid < t.k1 ? static_cast<T>(rd) : T(0);
+  };
+  const int g = lane >> 4, l16 = lane & 15;
+  gload(t.k0);
+  for (int kb = t.k0; kb < t.k1; kb += KT) {
+    __syncthreads();
+    store(kb);
+    __syncthreads();
+    if (kb + KT < t.k1) gload(kb + KT);
+#pragma unroll
+    for (int h = 0; h < KT / 16; ++h)
+#pragma unroll
+      for (int p = 0; p < 4 / E; ++p) {
+        const int kk = 16 * h + 4 * g + E * p;
+        L16 av[4], bv[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] = *reinterpret_cast<const L16*>(&As[wm * 64 + bi * 16 + l16][kk]);
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) bv[bj] = *reinterpret_cast<const L16*>(&Bs[wn * 64 + bj * 16 + l16][kk]);
+        const L16 dv = *reinterpret_cast<const L16*>(&Ds[kk]);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] *= dv;
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(bv[bj][s], av[bi][s], acc[bi][bj]);
+      }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int row = t.r0 + wm * 64 + bi * 16 + l16;
+    const int rowc = min(row, d.m - 1);
+    T cv[4][4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int colc = min(t.c0 + wn * 64 + bj * 16 + M::orow(lane, r), t.cend - 1);
+        cv[bj][r] = F[rowc + static_cast<int64_t>(colc) * m];
+      }
+    if (row >= d.m) continue;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = t.c0 + wn * 64 + bj * 16 + M::orow(lane, r);
+        if (col < t.cend) F[row + static_cast<int64_t>(col) * m] = cv[bj][r] - acc[bi][bj][r];
+      }
+  }
+}
+
+}  // namespace rf

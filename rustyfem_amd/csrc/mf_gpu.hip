@@ -20,6 +20,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "gemm_kernel.hpp"
 #include "gpu.hpp"
 
 namespace rf {
@@ -40,32 +41,6 @@ std::string& last_error() {
   } while (0)
 
 namespace {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-
-// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16], B[l/16][l%16]; output
-// register r of lane l is D[l/16 + 4r][l%16] (checked on gfx950).
-template <>
-struct Mfma<double> {
-  static constexpr int KT = 16;
-  using acc_t = v4d;
-  __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-  __device__ static int orow(int lane, int r) { return lane / 16 + 4 * r; }
-};
-
-// v_mfma_f32_16x16x4f32: same operand layout; output register r of lane l is
-// D[4(l/16) + r][l%16] (checked on gfx950).
-template <>
-struct Mfma<float> {
-  static constexpr int KT = 32;
-  using acc_t = v4f;
-  __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  __device__ static int orow(int lane, int r) { return 4 * (lane / 16) + r; }
-};
 
 // ---------------------------------------------------------------- assembly
 
@@ -611,101 +586,6 @@ __global__ __launch_bounds__(256) void k_prow(const FrontDesc* __restrict__ fd, 
       for (int r = 0; r < 4; ++r) {
         const int q = bj * 16 + M::orow(lane, r);
         if (q < jb) F[row + static_cast<int64_t>(t.j0 + q) * m] = T(o[bj][r] * dinv[q]);
-      }
-  }
-}
-
-// C[r0:r0+128, c0:min(c0+TN, cend)) -= (L[r0:, k0:k1] D) L[c0:, k0:k1]^T on MFMA.
-// Four waves tile 128 x TN (TN = 64: 4 x 1 waves of 32 x 64; TN = 128: 2 x 2
-// waves of 64 x 64).  The MFMA row index runs over C's columns so that lanes
-// 0..15 of an output register hold 16 consecutive rows of the column-major
-// front (coalesced read-modify-write).  The next K tile is prefetched into
-// registers while the current one is multiplied from LDS.
-template <typename T, int TN>
-__global__ __launch_bounds__(256) void k_gemm(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
-                                              T* __restrict__ arena, const double* __restrict__ dpiv) {
-  using M = Mfma<T>;
-  constexpr int KT = M::KT;
-  constexpr int PAD = 4;
-  constexpr int WN = TN / 64;
-  constexpr int WM = 4 / WN;
-  constexpr int BI = GT_M / WM / 16;
-  constexpr int BJ = 4;
-  constexpr int NA = GT_M * KT / 256;
-  constexpr int NBL = TN * KT / 256;
-  const GemmTile t = tiles[blockIdx.x];
-  const FrontDesc d = fd[t.front];
-  __shared__ T As[KT][GT_M + PAD];
-  __shared__ T Bs[KT][TN + PAD];
-  __shared__ T ds[KT];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
-  T* F = arena + d.off;
-  const int64_t m = d.m;
-  typename M::acc_t acc[BI][BJ];
-#pragma unroll
-  for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = typename M::acc_t{0, 0, 0, 0};
-  T ra[NA], rb[NBL], rd = T(0);
-  auto gload = [&](int kb) {
-#pragma unroll
-    for (int s = 0; s < NA; ++s) {
-      const int idx = tid + 256 * s;
-      const int i = idx % GT_M, kk = idx / GT_M;
-      const int r = t.r0 + i, c = kb + kk;
-      ra[s] = (r < d.m && c < t.k1) ? F[r + c * m] : T(0);
-    }
-#pragma unroll
-    for (int s = 0; s < NBL; ++s) {
-      const int idx = tid + 256 * s;
-      const int j = idx % TN, kk = idx / TN;
-      const int r = t.c0 + j, c = kb + kk;
-      rb[s] = (r < t.cend && c < t.k1) ? F[r + c * m] : T(0);
-    }
-    if (tid < KT) rd = (kb + tid < t.k1) ? static_cast<T>(dpiv[d.start + kb + tid]) : T(0);
-  };
-  gload(t.k0);
-  for (int kb = t.k0; kb < t.k1; kb += KT) {
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < NA; ++s) {
-      const int idx = tid + 256 * s;
-      As[idx / GT_M][idx % GT_M] = ra[s];
-    }
-#pragma unroll
-    for (int s = 0; s < NBL; ++s) {
-      const int idx = tid + 256 * s;
-      Bs[idx / TN][idx % TN] = rb[s];
-    }
-    if (tid < KT) ds[tid] = rd;
-    __syncthreads();
-    if (kb + KT < t.k1) gload(kb + KT);
-#pragma unroll
-    for (int kk = 0; kk < KT; kk += 4) {
-      const int kr = kk + lane / 16;
-      const T dk = ds[kr];
-      T a[BJ], b[BI];
-#pragma unroll
-      for (int bj = 0; bj < BJ; ++bj) a[bj] = Bs[kr][wn * 64 + bj * 16 + lane % 16];
-#pragma unroll
-      for (int bi = 0; bi < BI; ++bi) b[bi] = As[kr][wm * (GT_M / WM) + bi * 16 + lane % 16] * dk;
-#pragma unroll
-      for (int bi = 0; bi < BI; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = M::op(a[bj], b[bi], acc[bi][bj]);
-    }
-  }
-#pragma unroll
-  for (int bi = 0; bi < BI; ++bi) {
-    const int row = t.r0 + wm * (GT_M / WM) + bi * 16 + lane % 16;
-    if (row >= d.m) continue;
-#pragma unroll
-    for (int bj = 0; bj < BJ; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = t.c0 + wn * 64 + bj * 16 + M::orow(lane, r);
-        if (col < t.cend) F[row + col * m] -= acc[bi][bj][r];
       }
   }
 }
@@ -1883,10 +1763,8 @@ class Impl final : public GpuFactor {
       }
       HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
     }
-    if (ph.kind == PhaseKind::Gemm64)
-      launch(k_gemm<T, 64>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
-    else
-      launch(k_gemm<T, 128>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    if (ph.kind != PhaseKind::Gemm128) throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
+    launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
