@@ -21,6 +21,7 @@
 #include <stdexcept>
 
 #include "gemm_kernel.hpp"
+#include "panel_kernels.hpp"
 #include "gpu.hpp"
 
 namespace rf {
@@ -208,13 +209,6 @@ __global__ __launch_bounds__(256) void k_trsm(const FrontDesc* __restrict__ fd, 
   }
 }
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
-  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-
 // One inner pivot step of one front for one tile of rows (PanelTile):
 //  1. left-looking update from the outer block's columns [J0, j0) of the
 //     diagonal block (C_BB) and of the tile's rows (C_R), FP64 MFMA;
@@ -239,12 +233,16 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
   const bool diag_tile = t.r0 == t.j0;
   T* F = arena + d.off;
   const int64_t m = d.m;
-  __shared__ double Ad[NB][NB + 1];   // diagonal block [col][row]; later L / D
-  __shared__ double Ar[NB][NB + 1];   // the tile's rows [col][row]; later W
-  __shared__ double Vs[NB][NB + 1];   // L^-1 [col][row]
-  __shared__ double Ys[KS][NB + 2];   // L[j0 + q, k] d_k
-  __shared__ double Y0[KS][NB + 2];   // L[j0 + i, k]
+  __shared__ double Ad[NB][NB + 1];   // diagonal block [col][row]
+  __shared__ double Ar[NB][NB + 1];   // the tile's rows [col][row]
+  __shared__ __attribute__((aligned(16))) double Vs[NB][NB + 2];        // L^-1 [col][row]
+  __shared__ __attribute__((aligned(16))) double YY[2][KS][NB + 2];     // Ys, Y0; later L
   __shared__ double Xs[KS][NB + 2];   // L[r0 + i, k]
+  __shared__ double cbuf[NB], dd[NB];
+  double(*Ys)[NB + 2] = YY[0];        // L[j0 + q, k] d_k
+  double(*Y0)[NB + 2] = YY[1];        // L[j0 + i, k]
+  static_assert(2 * KS >= NB, "L fits in the update buffers");
+  double(*Ls)[NB + 2] = reinterpret_cast<double(*)[NB + 2]>(&YY[0][0][0]);
   M::acc_t cb[4], cr[4];
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj) {
@@ -298,40 +296,8 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
     }
   __syncthreads();
   if (wave == 0) {
-    double a[NB], v[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q) a[q] = Ad[q][lane];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j < jb) {
-        const double dj = readlane_d(a[j], j);
-        if (diag_tile && lane == 0) {
-          const int64_t eq = static_cast<int64_t>(d.start) + t.j0 + j;
-          const double orig = adiag[eq];
-          if (!(orig > 0.0) || !(dj > 1e-12 * orig)) atomicMin(err, static_cast<unsigned long long>(eq));
-        }
-        const double l = a[j] / dj;
-#pragma unroll
-        for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, readlane_d(a[q], j), a[q]);
-        if (lane > j) a[j] = l;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) v[q] = q == lane ? 1.0 : 0.0;
-#pragma unroll
-    for (int p = 0; p < NB; ++p) {
-      if (p < jb) {
-        const double lip = lane > p ? a[p] : 0.0;
-#pragma unroll
-        for (int q = 0; q <= p; ++q) v[q] = fma(-lip, readlane_d(v[q], p), v[q]);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const bool in = lane < jb && q < jb;
-      Ad[q][lane] = in && q <= lane ? a[q] : 0.0;  // L strictly below, D on the diagonal
-      Vs[q][lane] = in && q <= lane ? v[q] : 0.0;  // L^-1[lane][q]
-    }
+    const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
+    block_ldlt(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, diag_tile ? err : nullptr);
   }
   __syncthreads();
   if (diag_tile) {
@@ -340,13 +306,13 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
     // block's inverse (V) and D instead of L_BB.
     double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
     for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
-    if (tid < jb) dpiv[d.start + t.j0 + tid] = Ad[tid][tid];
+    if (tid < jb) dpiv[d.start + t.j0 + tid] = dd[tid];
     return;
   }
   // W[p][q] = L^-1[q][p] / d_q, kept in Ad's storage as Wd[p][q]
   double (*Wd)[NB + 1] = Ad;
   __shared__ double dinv[NB];
-  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / Ad[tid][tid] : 0.0;
+  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dd[tid] : 0.0;
   __syncthreads();
   for (int idx = tid; idx < NB * NB; idx += 256) {
     const int q = idx % NB, p = idx / NB;
@@ -364,221 +330,6 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
     for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Wd[kr][bj * 16 + lane % 16], b, o[bj]);
   }
   const int row = t.r0 + wave * 16 + lane % 16;
-  if (row < d.m) {
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = bj * 16 + M::orow(lane, r);
-        if (q < jb) F[row + static_cast<int64_t>(t.j0 + q) * m] = T(o[bj][r] * dinv[q]);
-      }
-  }
-}
-
-// The inner pivot step split in two launches.  k_pdiag (one workgroup per
-// front): left-looking update of the 64x64 pivot block from the outer block's
-// columns [J0, j0), LDL^T with the pivot test and the inverse in the
-// registers of one wave; writes D and V = L_BB^-1.  k_prow (one workgroup per
-// 64-row tile below the block): left-looking update of the tile's rows, then
-// L_R = A_R L_BB^-T D^-1 on MFMA with W built from V and D.  The K chunks
-// are prefetched into registers while the previous chunk is multiplied.
-template <typename T>
-__global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
-                                               const T* __restrict__ arena, double* __restrict__ dpiv,
-                                               const double* __restrict__ adiag, double* __restrict__ vbuf,
-                                               unsigned long long* __restrict__ err) {
-  using M = Mfma<double>;
-  constexpr int KS = 32;
-  constexpr int NL = KS * NB / 256;
-  const PanelTile t = tiles[blockIdx.x];
-  const FrontDesc d = fd[t.front];
-  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const T* F = arena + d.off;
-  const int64_t m = d.m;
-  __shared__ double Ad[NB][NB + 1];
-  __shared__ double Vs[NB][NB + 1];
-  __shared__ double Ys[KS][NB + 2];
-  __shared__ double Y0[KS][NB + 2];
-  M::acc_t cb[4];
-#pragma unroll
-  for (int bj = 0; bj < 4; ++bj) cb[bj] = M::acc_t{0, 0, 0, 0};
-  double py[NL], pd[NL];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int s = 0; s < NL; ++s) {
-      const int e = tid + 256 * s, i = e % NB, k = k0 + e / NB;
-      const bool kin = k < t.j0;
-      py[s] = (kin && i < jb) ? static_cast<double>(F[t.j0 + i + k * m]) : 0.0;
-      pd[s] = kin ? dpiv[d.start + k] : 0.0;
-    }
-  };
-  if (t.J0 < t.j0) gload(t.J0);
-  for (int k0 = t.J0; k0 < t.j0; k0 += KS) {
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < NL; ++s) {
-      const int e = tid + 256 * s, i = e % NB, kk = e / NB;
-      Y0[kk][i] = py[s];
-      Ys[kk][i] = py[s] * pd[s];
-    }
-    __syncthreads();
-    if (k0 + KS < t.j0) gload(k0 + KS);
-#pragma unroll
-    for (int kk = 0; kk < KS; kk += 4) {
-      const int kr = kk + lane / 16;
-      const double b0 = Y0[kr][wave * 16 + lane % 16];
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj) cb[bj] = M::op(Ys[kr][bj * 16 + lane % 16], b0, cb[bj]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = bj * 16 + M::orow(lane, r);
-      const int i = wave * 16 + lane % 16;
-      double fb = 0.0;
-      if (i < jb && q < jb) {
-        const int hi = i > q ? i : q, lo = i > q ? q : i;
-        fb = static_cast<double>(F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m]) - cb[bj][r];
-      }
-      Ad[q][i] = fb;
-    }
-  __syncthreads();
-  if (wave == 0) {
-    double a[NB], v[NB];
-    // the original diagonal of this lane's pivot, for the test after the elimination
-    const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
-#pragma unroll
-    for (int q = 0; q < NB; ++q) a[q] = Ad[q][lane];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j < jb) {
-        const double dj = readlane_d(a[j], j);
-        const double l = a[j] / dj;
-#pragma unroll
-        for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, readlane_d(a[q], j), a[q]);
-        if (lane > j) a[j] = l;
-      }
-    }
-    // pivot test (src/sparse.rs:528-531): lane j holds D_j in a[j]; the first
-    // failing equation wins through the minimum
-    double dl = 0.0;
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (q == lane) dl = a[q];
-    if (lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
-      atomicMin(err, static_cast<unsigned long long>(static_cast<int64_t>(d.start) + t.j0 + lane));
-#pragma unroll
-    for (int q = 0; q < NB; ++q) v[q] = q == lane ? 1.0 : 0.0;
-#pragma unroll
-    for (int p = 0; p < NB; ++p) {
-      if (p < jb) {
-        const double lip = lane > p ? a[p] : 0.0;
-#pragma unroll
-        for (int q = 0; q <= p; ++q) v[q] = fma(-lip, readlane_d(v[q], p), v[q]);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const bool in = lane < jb && q < jb;
-      Ad[q][lane] = in && q <= lane ? a[q] : 0.0;
-      Vs[q][lane] = in && q <= lane ? v[q] : 0.0;
-    }
-  }
-  __syncthreads();
-  double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
-  for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
-  if (tid < jb) dpiv[d.start + t.j0 + tid] = Ad[tid][tid];
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_prow(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
-                                              T* __restrict__ arena, const double* __restrict__ dpiv,
-                                              const double* __restrict__ vbuf) {
-  using M = Mfma<double>;
-  constexpr int KS = 32;
-  constexpr int NL = KS * NB / 256;
-  const PanelTile t = tiles[blockIdx.x];
-  const FrontDesc d = fd[t.front];
-  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  T* F = arena + d.off;
-  const int64_t m = d.m;
-  __shared__ double Ys[KS][NB + 2];   // L[j0 + q, k] d_k
-  __shared__ double Xs[KS][NB + 2];   // L[r0 + i, k]
-  __shared__ double Wd[NB][NB + 1];   // W[p][q] = L_BB^-1[q][p]
-  __shared__ double dinv[NB];
-  M::acc_t cr[4];
-#pragma unroll
-  for (int bj = 0; bj < 4; ++bj) cr[bj] = M::acc_t{0, 0, 0, 0};
-  // the pivot block's inverse and D (written by the previous launch) are
-  // requested first, so that their latency hides behind the update loop
-  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
-  double wv[NB * NB / 256];
-#pragma unroll
-  for (int s = 0; s < NB * NB / 256; ++s) wv[s] = vb[tid + 256 * s];
-  const double dv = tid < jb ? dpiv[d.start + t.j0 + tid] : 1.0;
-  double py[NL], px[NL];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int s = 0; s < NL; ++s) {
-      const int e = tid + 256 * s, i = e % NB, k = k0 + e / NB;
-      const bool kin = k < t.j0;
-      const double y = (kin && i < jb) ? static_cast<double>(F[t.j0 + i + k * m]) : 0.0;
-      py[s] = kin ? y * dpiv[d.start + k] : 0.0;
-      px[s] = (kin && t.r0 + i < d.m) ? static_cast<double>(F[t.r0 + i + k * m]) : 0.0;
-    }
-  };
-  if (t.J0 < t.j0) gload(t.J0);
-  for (int k0 = t.J0; k0 < t.j0; k0 += KS) {
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < NL; ++s) {
-      const int e = tid + 256 * s, i = e % NB, kk = e / NB;
-      Ys[kk][i] = py[s];
-      Xs[kk][i] = px[s];
-    }
-    __syncthreads();
-    if (k0 + KS < t.j0) gload(k0 + KS);
-#pragma unroll
-    for (int kk = 0; kk < KS; kk += 4) {
-      const int kr = kk + lane / 16;
-      const double bx = Xs[kr][wave * 16 + lane % 16];
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj) cr[bj] = M::op(Ys[kr][bj * 16 + lane % 16], bx, cr[bj]);
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < NB * NB / 256; ++s) {
-    const int idx = tid + 256 * s, q = idx % NB, p = idx / NB;
-    Wd[p][q] = q < jb ? wv[s] : 0.0;
-  }
-  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dv : 0.0;
-  // A_R = F_R - C_R stays in the registers that hold C_R: at TRSM step s the
-  // lane's B operand is column 4s + lane/16 of its row, which is register
-  // (s / 4, s % 4) of cr
-  const int i = wave * 16 + lane % 16;
-  const int row = t.r0 + i;
-  double ar[16];
-#pragma unroll
-  for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = bj * 16 + M::orow(lane, r);
-      ar[bj * 4 + r] = (row < d.m && q < jb) ? static_cast<double>(F[row + static_cast<int64_t>(t.j0 + q) * m]) - cr[bj][r]
-                                            : 0.0;
-    }
-  __syncthreads();
-  M::acc_t o[4];
-#pragma unroll
-  for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
-#pragma unroll
-  for (int s4 = 0; s4 < 16; ++s4) {
-    const int kr = 4 * s4 + lane / 16;
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Wd[kr][bj * 16 + lane % 16], ar[s4], o[bj]);
-  }
   if (row < d.m) {
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)

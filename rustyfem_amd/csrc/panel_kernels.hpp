@@ -1,0 +1,326 @@
+// Inner pivot step of the multifrontal LDL^T on gfx950: one 64-column pivot
+// block B = [j0, j0 + jb) of each front, inside an outer block [J0, J0 + 256).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mf.hpp"
+#include "mfma.hpp"
+
+namespace rf {
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// LDL^T of the jb x jb pivot block (jb <= 64) and the inverse of its unit
+// lower factor, by wave 0 alone (lane i owns row i in registers).
+//   in:  A[q][i] = A_BB[i][q], both triangles, zero outside jb x jb
+//   out: dout[i] = D_i; V[q][i] = L^-1[i][q] for q <= i < jb, zero elsewhere;
+//        the pivot test (src/sparse.rs:528-531) against orig = the lane's
+//        original diagonal, the first failing equation eq0 + i wins in *err
+//        (no test when err is null)
+// Step j broadcasts column j of the trailing matrix through LDS (cbuf, one
+// write per lane, uniform-address reads): by symmetry it is row j, the
+// operand of the right-looking update of every row.  The inverse is built
+// one column per lane (lane q solves L x = e_q) from L staged in LDS.  A
+// wave executes its LDS instructions in order, so the exchanges need no
+// barrier; the wave barriers only keep the compiler from reordering them.
+__device__ __forceinline__ void block_ldlt(const double (*A)[NB + 1], double (*Ls)[NB + 2], double (*V)[NB + 2],
+                                  double* cbuf, double* dout, int jb, double orig, int64_t eq0,
+                                  unsigned long long* err) {
+  const int lane = threadIdx.x & 63;
+  double a[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) a[q] = A[q][lane];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < jb) {
+      cbuf[lane] = a[j];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // every read of the step is issued before the first product
+      double w[NB];
+#pragma unroll
+      for (int q = j; q < NB; ++q) w[q] = cbuf[q];
+      __builtin_amdgcn_sched_barrier(0);
+      const double l = a[j] / w[j];
+#pragma unroll
+      for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, w[q], a[q]);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane > j) a[j] = l;
+    }
+  }
+  // lane j holds D_j in a[j]
+  double dl = 0.0;
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+    if (q == lane) dl = a[q];
+  if (err != nullptr && lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
+    atomicMin(err, static_cast<unsigned long long>(eq0 + lane));
+  dout[lane] = lane < jb ? dl : 0.0;
+#pragma unroll
+  for (int p = 0; p < NB; ++p) Ls[lane][p] = (p < lane && lane < jb) ? a[p] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = i == lane ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 1; i < NB; ++i) {
+    if (i < jb) {
+      double w[NB];
+#pragma unroll
+      for (int p = 0; p < i; ++p) w[p] = Ls[i][p];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int p = 0; p < i; ++p) x[i] = fma(-w[p], x[p], x[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) V[lane][i] = (lane < jb && i < jb && i >= lane) ? x[i] : 0.0;
+}
+
+// Columns of the left-looking update staged in LDS per pass: the update of a
+// pivot block reads at most NBO - NB = 192 earlier columns of its outer
+// block; one pass in FP32 (50 KB of LDS), two in FP64.
+template <typename T>
+struct PanelPass {
+  static constexpr int PK = 96 * 8 / static_cast<int>(sizeof(T));
+};
+
+// The pass's global loads are issued together, before the LDS of the previous
+// pass is released, so that one memory latency is exposed per pass.  Rows
+// beyond jb (or beyond the front) are clamped to a valid row: they only reach
+// update entries that are discarded.  The MFMA order over k is that of one
+// k-step of 4 at a time (lane group g = lane / 16 takes k = 4 t + g at step
+// t), the order of the fused panel kernel, so both give identical factors.
+//
+// k_pdiag (one workgroup per front): C_BB = Y D Y^T with Y = L[B, J0:j0],
+// A_BB = F_BB - C_BB, then block_ldlt in wave 0; writes D and V = L_BB^-1
+// (the arena's pivot block stays as assembled).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
+                                               const T* __restrict__ arena, double* __restrict__ dpiv,
+                                               const double* __restrict__ adiag, double* __restrict__ vbuf,
+                                               unsigned long long* __restrict__ err) {
+  using M = Mfma<double>;
+  constexpr int PK = PanelPass<T>::PK;
+  constexpr int NS = PK / 4;
+  constexpr int YBYTES = PK * (NB + 2) * static_cast<int>(sizeof(T)) + PK * 8;
+  constexpr int ABYTES = NB * (NB + 1) * 8;  // Ad, then L (pitch NB + 2) in the same bytes
+  constexpr int LBYTES = NB * (NB + 2) * 8;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[YBYTES > LBYTES ? YBYTES : LBYTES];
+  __shared__ __attribute__((aligned(16))) double Vs[NB][NB + 2];  // Vs[q][i] = L^-1[i][q]
+  __shared__ double cbuf[NB], dd[NB];
+  static_assert(ABYTES <= LBYTES, "Ad and L share storage");
+  T(*Yk)[NB + 2] = reinterpret_cast<T(*)[NB + 2]>(smem);                            // Yk[k][i] = L[j0 + i, p0 + k]
+  double* Ds = reinterpret_cast<double*>(smem + PK * (NB + 2) * sizeof(T));          // D[p0 + k]
+  double(*Ad)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);                  // after the update
+  double(*Ls)[NB + 2] = reinterpret_cast<double(*)[NB + 2]>(smem);                  // after Ad is in registers
+  const PanelTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const T* F = arena + d.off;
+  const int64_t m = d.m;
+  const int i = wave * 16 + (lane & 15);
+  // the original diagonal of lane i's pivot (for the test after the
+  // elimination) and the block's own entries (lower triangle, mirrored)
+  const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
+  T fb[4][4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      const int hi = min(max(i, q), jb - 1), lo = min(min(i, q), jb - 1);
+      fb[bj][r] = F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m];
+    }
+  M::acc_t cb[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) cb[bj] = M::acc_t{0, 0, 0, 0};
+  const int li = tid & 63, kq = tid >> 6;
+  const T* Fy = F + t.j0 + min(li, jb - 1);
+  for (int p0 = t.J0; p0 < t.j0; p0 += PK) {
+    const int kn = min(PK, t.j0 - p0);
+    T ry[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (4 * s < kn) ry[s] = Fy[static_cast<int64_t>(p0 + kq + 4 * s) * m];
+    const double rd = tid < kn ? dpiv[d.start + p0 + tid] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (4 * s < kn) Yk[kq + 4 * s][li] = ry[s];
+    if (tid < kn) Ds[tid] = rd;
+    __syncthreads();
+    // four k-steps per group; the next group's operands are read before the
+    // current group's products
+    T ya[2][4][4], yb[2][4];
+    double yd[2][4];
+    auto rd4 = [&](int buf, int k4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kr = k4 + 4 * u + lane / 16;
+        yd[buf][u] = Ds[kr];
+        yb[buf][u] = Yk[kr][i];
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) ya[buf][u][bj] = Yk[kr][bj * 16 + (lane & 15)];
+      }
+    };
+    rd4(0, 0);
+#pragma unroll
+    for (int gq = 0; gq < PK / 16; ++gq) {
+      if (16 * gq < kn) {
+        if (16 * gq + 16 < kn) rd4((gq + 1) & 1, 16 * gq + 16);
+        __builtin_amdgcn_sched_barrier(0);
+        const int cur = gq & 1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double b0 = static_cast<double>(yb[cur][u]);
+#pragma unroll
+          for (int bj = 0; bj < 4; ++bj)
+            cb[bj] = M::op(static_cast<double>(ya[cur][u][bj]) * yd[cur][u], b0, cb[bj]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      Ad[q][i] = (i < jb && q < jb) ? static_cast<double>(fb[bj][r]) - cb[bj][r] : 0.0;
+    }
+  __syncthreads();
+  if (wave == 0) {
+    block_ldlt(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, err);
+  }
+  __syncthreads();
+  double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+  for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
+  if (tid < jb) dpiv[d.start + t.j0 + tid] = dd[tid];
+}
+
+// k_prow (one workgroup per 64-row tile below the block): C_R = X D Y^T with
+// X = L[r0:r0+64, J0:j0] (in registers, in the MFMA operand layout), then
+// L_R = (F_R - C_R) L_BB^-T D^-1 on MFMA with W built from V and D.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_prow(
+    const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles, T* __restrict__ arena,
+    const double* __restrict__ dpiv, const double* __restrict__ vbuf) {
+  using M = Mfma<double>;
+  constexpr int PK = PanelPass<T>::PK;
+  constexpr int NS = PK / 4;
+  constexpr int YBYTES = PK * (NB + 2) * static_cast<int>(sizeof(T)) + PK * 8;
+  constexpr int WBYTES = NB * (NB + 1) * 8;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[YBYTES > WBYTES ? YBYTES : WBYTES];
+  __shared__ double dinv[NB];
+  T(*Yk)[NB + 2] = reinterpret_cast<T(*)[NB + 2]>(smem);
+  double* Ds = reinterpret_cast<double*>(smem + PK * (NB + 2) * sizeof(T));
+  double(*Wd)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);  // W[p][q] = L_BB^-1[q][p], after the update
+  const PanelTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int jb = t.jb, tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  // the pivot block's inverse and D (written by the previous launch) first
+  const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
+  double wv[NB * NB / 256];
+#pragma unroll
+  for (int s = 0; s < NB * NB / 256; ++s) wv[s] = vb[tid + 256 * s];
+  const double dv = tid < jb ? dpiv[d.start + t.j0 + tid] : 1.0;
+  const int i = wave * 16 + (lane & 15);
+  const int row = t.r0 + i;
+  const int rowc = min(row, d.m - 1);
+  // A_R entries of this lane
+  T fa[4][4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = min(bj * 16 + M::orow(lane, r), jb - 1);
+      fa[bj][r] = F[rowc + static_cast<int64_t>(t.j0 + q) * m];
+    }
+  M::acc_t cr[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) cr[bj] = M::acc_t{0, 0, 0, 0};
+  const int li = tid & 63, kq = tid >> 6, g = lane / 16;
+  const T* Fy = F + t.j0 + min(li, jb - 1);
+  const T* Fx = F + rowc;
+  for (int p0 = t.J0; p0 < t.j0; p0 += PK) {
+    const int kn = min(PK, t.j0 - p0);
+    T ry[NS], rx[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (4 * s < kn) {
+        ry[s] = Fy[static_cast<int64_t>(p0 + kq + 4 * s) * m];
+        rx[s] = Fx[static_cast<int64_t>(p0 + 4 * s + g) * m];
+      }
+    const double rd = tid < kn ? dpiv[d.start + p0 + tid] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (4 * s < kn) Yk[kq + 4 * s][li] = ry[s];
+    if (tid < kn) Ds[tid] = rd;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (4 * s < kn) {
+        const int kr = 4 * s + g;
+        const double dk = Ds[kr];
+        const double bx = static_cast<double>(rx[s]);
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+          cr[bj] = M::op(static_cast<double>(Yk[kr][bj * 16 + (lane & 15)]) * dk, bx, cr[bj]);
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NB * NB / 256; ++s) {
+    const int idx = tid + 256 * s, q = idx % NB, p = idx / NB;
+    Wd[p][q] = q < jb ? wv[s] : 0.0;
+  }
+  if (tid < NB) dinv[tid] = tid < jb ? 1.0 / dv : 0.0;
+  // A_R = F_R - C_R stays in the registers that hold C_R: at TRSM step s the
+  // lane's B operand is column 4s + lane/16 of its row, which is register
+  // (s / 4, s % 4) of cr
+  double ar[16];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = bj * 16 + M::orow(lane, r);
+      ar[bj * 4 + r] = (row < d.m && q < jb) ? static_cast<double>(fa[bj][r]) - cr[bj][r] : 0.0;
+    }
+  __syncthreads();
+  M::acc_t o[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) o[bj] = M::acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s4 = 0; s4 < 16; ++s4) {
+    const int kr = 4 * s4 + lane / 16;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) o[bj] = M::op(Wd[kr][bj * 16 + (lane & 15)], ar[s4], o[bj]);
+  }
+  if (row < d.m) {
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = bj * 16 + M::orow(lane, r);
+        if (q < jb) F[row + static_cast<int64_t>(t.j0 + q) * m] = T(o[bj][r] * dinv[q]);
+      }
+  }
+}
+
+}  // namespace rf
