@@ -783,49 +783,86 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
   }
 }
 
+// 32-bit DPP move of both halves of a double (CTRL: a DPP control code)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// sum over the 16 lanes of a DPP row, returned in every lane of the row:
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x124>(v);
+  v += dpp_d<0x128>(v);
+  return v;
+}
+
 // partial[tile][j][c] = sum over the tile's rows of L[r, c] t[r][j], c in [c0, c1)
+// Wave w owns the tile's columns [64 w, 64 w + 64) in groups of four; in a
+// group, lane group cg = lane / 16 takes one column and its 16 lanes read 16
+// consecutive rows per load (four loads cover the 64 rows).  Every load of
+// the tile is issued before the first product; rows past the tile or the
+// front are clamped to a valid row and weighted by t = 0, columns past c1 are
+// clamped and not stored (their sums are written as 0).  The 16 partial sums
+// of a column are reduced across the DPP row.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__ fd, const SbPartTile* __restrict__ tiles,
                                                    const T* __restrict__ arena, const double* __restrict__ t,
                                                    int64_t tv, int nrhs, double* __restrict__ partial) {
+  static_assert(BC == 256 && PRT == 64, "four waves of 64 columns, 64 rows per tile");
   const SbPartTile tl = tiles[blockIdx.x];
   const FrontDesc d = fd[tl.front];
-  constexpr int SL = 32;
-  __shared__ T Ls[BC][SL + 1];
-  __shared__ double ts[SL][R];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rl = lane & 15, cg = lane >> 4;
   const int rend = min(tl.r0 + PRT, d.m);
   const T* F = arena + d.off;
   const double* tf = t + d.toff;
   double* out = partial + static_cast<int64_t>(blockIdx.x) * nrhs * BC;
+  int rowc[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) rowc[s] = min(tl.r0 + rl + 16 * s, rend - 1);
   for (int j0 = 0; j0 < nrhs; j0 += R) {
     const int nr = min(R, nrhs - j0);
-    double acc[R];
+    double tr[4][R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) acc[j] = 0.0;
-    for (int rs = tl.r0; rs < rend; rs += SL) {
-      __syncthreads();
-#pragma unroll 4
-      for (int e = tid; e < SL * BC; e += 256) {
-        const int rr = e % SL, cc = e / SL;
-        const int r = rs + rr, c = tl.c0 + cc;
-        Ls[cc][rr] = (r < rend && c < tl.c1) ? F[r + static_cast<int64_t>(c) * d.m] : T(0);
-      }
-      for (int e = tid; e < SL * R; e += 256) {
-        const int rr = e % SL, j = e / SL;
-        ts[rr][j] = (rs + rr < rend && j < nr) ? tf[(j0 + j) * tv + rs + rr] : 0.0;
-      }
-      __syncthreads();
-#pragma unroll 8
-      for (int rr = 0; rr < SL; ++rr) {
-        const double l = static_cast<double>(Ls[tid][rr]);
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int j = 0; j < R; ++j) acc[j] = fma(l, ts[rr][j], acc[j]);
+      for (int j = 0; j < R; ++j) {
+        const int row = tl.r0 + rl + 16 * s;
+        tr[s][j] = (row < rend && j < nr) ? tf[(j0 + j) * tv + row] : 0.0;
       }
+    T lv[16][4];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int cc = min(tl.c0 + 64 * wave + 4 * q + cg, tl.c1 - 1);
+      const T* col = F + static_cast<int64_t>(cc) * d.m;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) lv[q][s] = col[rowc[s]];
     }
 #pragma unroll
-    for (int j = 0; j < R; ++j)
-      if (j < nr) out[(j0 + j) * BC + tid] = acc[j];
+    for (int q = 0; q < 16; ++q) {
+      double acc[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        acc[j] = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc[j] = fma(static_cast<double>(lv[q][s]), tr[s][j], acc[j]);
+        acc[j] = row16_sum(acc[j]);
+      }
+      const int cl = 64 * wave + 4 * q + cg;
+      if (rl == 0) {
+        const bool in = tl.c0 + cl < tl.c1;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (j < nr) out[(j0 + j) * BC + cl] = in ? acc[j] : 0.0;
+      }
+    }
   }
 }
 
