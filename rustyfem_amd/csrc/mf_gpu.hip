@@ -82,15 +82,27 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
     my[tid - EA_T] = y < c.u ? cm[y] : 0;
   }
   __syncthreads();
-  const int lx = tid % EA_T;
+  // all reads of the thread's 16 entries are issued before its writes (from
+  // clamped, valid addresses; only entries with y <= x < u are written)
+  constexpr int NY = EA_T * EA_T / 256;
+  const int lx = tid % EA_T, ly0 = tid / EA_T;
   const int x = t.x0 + lx;
-  if (x >= c.u) return;
-  const T* src = arena + c.off + (c.k + x) + static_cast<int64_t>(c.k) * c.m;
+  const int xc = min(x, c.u - 1);
+  const T* src = arena + c.off + (c.k + xc) + static_cast<int64_t>(c.k) * c.m;
   T* dst = arena + p.off + mx[lx];
-  for (int ly = tid / EA_T; ly < EA_T; ly += 256 / EA_T) {
-    const int y = t.y0 + ly;
-    if (y > x) break;
-    dst[static_cast<int64_t>(my[ly]) * p.m] += src[static_cast<int64_t>(y) * c.m];
+  T sv[NY], dv[NY];
+#pragma unroll
+  for (int k = 0; k < NY; ++k) {
+    const int ly = ly0 + k * (256 / EA_T);
+    const int yc = min(t.y0 + ly, xc);
+    sv[k] = src[static_cast<int64_t>(yc) * c.m];
+    dv[k] = dst[static_cast<int64_t>(my[min(ly, yc - t.y0)]) * p.m];
+  }
+  if (x >= c.u) return;
+#pragma unroll
+  for (int k = 0; k < NY; ++k) {
+    const int ly = ly0 + k * (256 / EA_T);
+    if (t.y0 + ly <= x) dst[static_cast<int64_t>(my[ly]) * p.m] = dv[k] + sv[k];
   }
 }
 
@@ -767,7 +779,7 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (valid) {
-#pragma unroll 8
+#pragma unroll 16
       for (int c = cb; c < ce; ++c) {
         const double l = static_cast<double>(F[static_cast<int64_t>(c) * d.m]);
 #pragma unroll
