@@ -13,6 +13,9 @@ same structure:
   src/elements/continuum2d.rs:24-27, 35-80;
 * C3D8 (2x2x2 Gauss, volumetric B-bar): src/elements/continuum3d.rs:1-6,
   src/elements/common.rs:142-202;
+* C3D4 (linear tetrahedron, one point of weight 1/6): src/elements/solid_higher.rs:109,
+  the isoparametric map of src/elements/solid_higher.rs:144-190 and the B
+  matrix of src/elements/common.rs:142-174;
 * isotropic D matrices: src/materials/elastic.rs:47-70;
 * the symmetric part 0.5(ke + ke^T) scattered into the lower triangle:
   src/fem/assembly.rs:356-401.
@@ -426,6 +429,176 @@ def hetero_block(nx, log_std=1.5, corr_len=0.1, seed=4, e=210e9, nu=0.3, body=1e
     lognormal Young's modulus per element, clamped base z=0, body load along -z."""
     field = lognormal_field((nx, nx, nx), log_std, corr_len, seed)
     return hex_block(nx, e=e, nu=nu, escale=field.ravel(), clamp="z0", load="body_z", body=body)
+
+
+# Kuhn split of a cube into six tetrahedra around the diagonal from corner 0 to
+# corner 7 (corner bits: x = 1, y = 2, z = 4); every tetrahedron follows one
+# monotone path 0 -> 7, so all cubes of a grid split compatibly and a node's
+# neighbours are the 14 offsets d and -d with d in {0,1}^3 \ {0}.
+KUHN = ((0, 1, 3, 7), (0, 1, 5, 7), (0, 2, 3, 7), (0, 2, 6, 7), (0, 4, 5, 7), (0, 4, 6, 7))
+TET_DN = np.array([[-1.0, -1.0, -1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+def tet4_stiffness(x: np.ndarray, e: np.ndarray, nu: float):
+    """C3D4 element matrices for a batch of tetrahedra x (ne, 4, 3) with Young's
+    moduli e (ne,): K = V B^T D B with the constant B of the linear tetrahedron
+    (one integration point, weight 1/6 det J = V), symmetrized as the assembly
+    does.  Returns (K (ne, 12, 12), V (ne,)); raises on a non-positive det J
+    (src/elements/solid_higher.rs:190)."""
+    j = x[:, 1:, :] - x[:, :1, :]                     # J[b][c] = d x_c / d xi_b
+    det = np.linalg.det(j)
+    if not np.all(det > 0):
+        raise ValueError("distorted tetrahedron (det J <= 0)")
+    dndx = np.matmul(TET_DN, np.swapaxes(np.linalg.inv(j), 1, 2))  # dN_a/dx_c = sum_r Jinv[c][r] dN_a/dxi_r
+    ne = x.shape[0]
+    b = np.zeros((ne, 6, 12))
+    for a in range(4):
+        c = 3 * a
+        d0, d1, d2 = dndx[:, a, 0], dndx[:, a, 1], dndx[:, a, 2]
+        b[:, 0, c] = d0
+        b[:, 1, c + 1] = d1
+        b[:, 2, c + 2] = d2
+        b[:, 3, c] = d1
+        b[:, 3, c + 1] = d0
+        b[:, 4, c] = d2
+        b[:, 4, c + 2] = d0
+        b[:, 5, c + 1] = d2
+        b[:, 5, c + 2] = d1
+    vol = det / 6.0
+    d = dmat_3d(1.0, nu)
+    k = np.matmul(np.swapaxes(b, 1, 2), np.matmul(d, b)) * (vol * e)[:, None, None]
+    return 0.5 * (k + np.swapaxes(k, 1, 2)), vol
+
+
+def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4, perturb=0.2, chunk=400000,
+              clamp="z0"):
+    """C3D4 block on nx^3 cubes of the unit cube, each split into six tetrahedra
+    (KUHN), interior vertices moved by U[-perturb h, perturb h]^3, a lognormal
+    Young's modulus per cube (log-std drawn from U[0.5, 2] when not given,
+    correlation length corr_len), clamped base z = 0 (clamp=None: no support),
+    body load along -z.
+    Nodes are numbered as the box mesher numbers them (x fastest)."""
+    rng = np.random.default_rng(seed)
+    if log_std is None:
+        log_std = float(rng.uniform(0.5, 2.0))
+    h = 1.0 / nx
+    n1 = nx + 1
+    nn = n1 ** 3
+    kk, jj, ii = np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij")
+    coords = np.stack([ii.ravel() * h, jj.ravel() * h, kk.ravel() * h], axis=1)
+    inner = ((ii > 0) & (ii < nx) & (jj > 0) & (jj < nx) & (kk > 0) & (kk < nx)).ravel()
+    coords[inner] += rng.uniform(-perturb * h, perturb * h, size=(int(inner.sum()), 3))
+    field = lognormal_field((nx, nx, nx), log_std, corr_len, int(rng.integers(1 << 31)))
+    # corner c of cube (i, j, k) is node (i + c&1) + n1 (j + (c>>1)&1) + n1^2 (k + (c>>2)&1)
+    ck, cj, ci = np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx), indexing="ij")
+    base = (ci + n1 * (cj + n1 * ck)).ravel()
+    corner = np.array([(c & 1) + n1 * ((c >> 1) & 1) + n1 * n1 * ((c >> 2) & 1) for c in range(8)])
+    tets = np.array(KUHN)
+    # orient every tetrahedron positively on the undistorted cube (a corner swap)
+    cube = np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)], dtype=np.float64)
+    for t in range(6):
+        if np.linalg.det(cube[tets[t, 1:]] - cube[tets[t, 0]]) < 0:
+            tets[t, [2, 3]] = tets[t, [3, 2]]
+    conn = (base[:, None, None] + corner[tets][None, :, :]).reshape(-1, 4)   # (6 nx^3, 4), cube-major
+    ecube = np.repeat(np.arange(nx ** 3), 6)
+    estiff = e * field.ravel()[ecube]
+    # stencil offsets sorted by node index: d in {-1,0,1}^3 with all components of one sign
+    offs = sorted({(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+                   if min(dz, dy, dx) >= 0 or max(dz, dy, dx) <= 0})
+    assert len(offs) == 15
+    oid = {o: i for i, o in enumerate(offs)}
+    # offset index of every ordered pair (a, b) of local nodes of each of the 6 tetrahedra
+    pair_off = np.zeros((6, 4, 4), dtype=np.int64)
+    cvec = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)])
+    for t in range(6):
+        for a in range(4):
+            for b_ in range(4):
+                dv = tuple(int(v) for v in cvec[tets[t, b_]] - cvec[tets[t, a]])
+                pair_off[t, a, b_] = oid[dv]
+    sten = np.zeros((nn * 15, 9))
+    f = np.zeros((nn, 3))
+    ne = conn.shape[0]
+    for s0 in range(0, ne, chunk):
+        s1 = min(ne, s0 + chunk)
+        cn = conn[s0:s1]
+        ke, vol = tet4_stiffness(coords[cn], estiff[s0:s1], nu)
+        tloc = np.arange(s0, s1) % 6
+        for a in range(4):
+            f[:, 2] -= np.bincount(cn[:, a], weights=body * vol / 4.0, minlength=nn)
+        idx = np.concatenate([cn[:, a] * 15 + pair_off[tloc, a, b_] for a in range(4) for b_ in range(4)])
+        blk = np.concatenate([ke[:, 3 * a:3 * a + 3, 3 * b_:3 * b_ + 3].reshape(-1, 9)
+                              for a in range(4) for b_ in range(4)])
+        for q in range(9):
+            sten[:, q] += np.bincount(idx, weights=blk[:, q], minlength=nn * 15)
+    sten = sten.reshape(nn, 15, 3, 3)
+    # exact symmetry: the block towards a lower-numbered neighbour is the transpose
+    # of that neighbour's block towards this node (offsets o and 14 - o are
+    # opposite), and diagonal blocks are symmetrized
+    zi, yi, xi = np.unravel_index(np.arange(nn), (n1, n1, n1))
+    for o in range(8, 15):
+        dz, dy, dx = offs[o]
+        z, y, x = zi + dz, yi + dy, xi + dx
+        ok = (z < n1) & (y < n1) & (x < n1)
+        src = np.nonzero(ok)[0]
+        dst = (x + n1 * (y + n1 * z))[ok]
+        sten[dst, 14 - o] = np.swapaxes(sten[src, o], -1, -2)
+    sten[:, 7] = 0.5 * (sten[:, 7] + np.swapaxes(sten[:, 7], -1, -2))
+    free = np.ones(nn, dtype=bool)
+    if clamp == "z0":
+        free[(kk == 0).ravel()] = False
+    elif clamp is not None:
+        raise ValueError(clamp)
+    n, indptr, indices, data = _stencil_csc(sten, offs, (n1, n1, n1), free)
+    loc = np.repeat(np.nonzero(free)[0], 3).astype(np.int64)
+    rhs = np.zeros((n, 1), order="F")
+    rhs[:, 0] = f[free].ravel()
+    return Problem(f"tet_c3d4_{nx}", n, nn, loc, coords, indptr, indices, data, rhs, ndof=3,
+                   meta={"nx": nx, "log_std": log_std, "seed": seed, "e": e, "nu": nu})
+
+
+def _stencil_csc(sten, offs, shape_n, free):
+    """CSC (both triangles) over free DOFs, node-major, from per-node 3x3 stencil
+    blocks sten[p, o] = K(p, p + offs[o]) with offsets sorted by neighbour index:
+    entry (row (q, r), column (p, c)) = sten[p, o(q - p)][c, r] (the matrix is
+    symmetric block-wise, sten[q, o(p - q)] = sten[p, o(q - p)]^T)."""
+    nz_, ny_, nx_ = shape_n
+    nn = nz_ * ny_ * nx_
+    zi, yi, xi = np.unravel_index(np.arange(nn), shape_n)
+    nbr = np.full((nn, len(offs)), -1, dtype=np.int64)
+    for o, (dz, dy, dx) in enumerate(offs):
+        z, y, x = zi + dz, yi + dy, xi + dx
+        ok = (z >= 0) & (z < nz_) & (y >= 0) & (y < ny_) & (x >= 0) & (x < nx_)
+        nbr[ok, o] = (x + nx_ * (y + ny_ * z))[ok]
+    fid = np.full(nn, -1, dtype=np.int64)
+    fid[free] = np.arange(int(free.sum()))
+    nbr_f = np.where(nbr >= 0, fid[np.maximum(nbr, 0)], -1)
+    valid = (nbr_f >= 0) & free[:, None]
+    fnodes = np.nonzero(free)[0]
+    nf = len(fnodes)
+    n = 3 * nf
+    cnt = valid[fnodes].sum(axis=1)
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.repeat(3 * cnt, 3), out=indptr[1:])
+    nnz = int(indptr[-1])
+    indices = np.empty(nnz, dtype=np.int64)
+    data = np.empty(nnz)
+    vm = valid[fnodes]
+    rank = np.cumsum(vm, axis=1) - 1
+    k = int(cnt.max()) if nf else 0
+    qc = np.full((nf, k), -1, dtype=np.int64)
+    sc = np.zeros((nf, k, 3, 3))
+    rr, cc = np.nonzero(vm)
+    qc[rr, rank[rr, cc]] = nbr_f[fnodes][rr, cc]
+    sc[rr, rank[rr, cc]] = sten[fnodes][rr, cc]
+    pos = indptr[:-1].reshape(nf, 3)
+    for c in range(3):
+        for j in range(k):
+            ok = qc[:, j] >= 0
+            for r in range(3):
+                idx = pos[ok, c] + 3 * j + r
+                indices[idx] = 3 * qc[ok, j] + r
+                data[idx] = sc[ok, j, c, r]
+    return n, indptr, indices, data
 
 
 def node_graph(p: Problem):
