@@ -98,7 +98,10 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
  * [6] device bytes held, [7] precision, [8] number of fronts, [9] levels,
  * [10] summed duration (ms) of the Schur-update GEMM launches, [11] their
  * count and [12] their algorithmic flops -- [10] and [11] are filled when the
- * environment variable RUSTYFEM_KERNEL_TIMING=1 was set at factorization. */
+ * environment variable RUSTYFEM_KERNEL_TIMING=1 was set at factorization;
+ * [13] time (ms) of the GEMM autotuning done before the first factorization
+ * (RUSTYFEM_AUTOTUNE=0 turns it off), [14] the GEMM launches it timed and
+ * [15] how many of them it moved to the AGPR-accumulator kernel. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */

@@ -484,7 +484,7 @@ void rustyfem_batch_info(const BatchHandle* h, double* out) {
 }
 
 void rustyfem_batch_stats(const BatchHandle* h, double* out, size_t len) {
-  double v[13] = {0};
+  double v[16] = {0};
   const rf::Timings& t = h->gpu->timings();
   v[0] = t.factor_ms;
   v[1] = t.solve_ms;
@@ -499,7 +499,10 @@ void rustyfem_batch_stats(const BatchHandle* h, double* out, size_t len) {
   v[10] = t.gemm_ms;
   v[11] = t.gemm_launches;
   v[12] = t.gemm_flops;
-  for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
+  v[13] = t.tune_ms;
+  v[14] = t.tuned_launches;
+  v[15] = t.agpr_launches;
+  for (size_t i = 0; i < len && i < 16; ++i) out[i] = v[i];
 }
 
 void rustyfem_free_batch(BatchHandle* h) { delete h; }
@@ -603,7 +606,7 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
 }
 
 void rustyfem_stats(const Handle* h, double* out, size_t len) {
-  double v[13] = {0};
+  double v[16] = {0};
   if (h->gpu) {
     const rf::Timings& t = h->gpu->timings();
     v[0] = t.factor_ms;
@@ -616,11 +619,14 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
     v[10] = t.gemm_ms;
     v[11] = t.gemm_launches;
     v[12] = t.gemm_flops;
+    v[13] = t.tune_ms;
+    v[14] = t.tuned_launches;
+    v[15] = t.agpr_launches;
   }
   v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
   v[8] = static_cast<double>(h->nfronts);
   v[9] = static_cast<double>(h->nlevels);
-  for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < 16; ++i) out[i] = v[i];
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }

@@ -27,12 +27,10 @@ namespace rf {
 // values reach only C entries that are not stored; columns beyond k1 are
 // clamped to k1 - 1 and their D is 0.  Column addresses are wave-uniform.
 // The C tile is read 16 entries at a time before it is written, so that the
-// reads of a group overlap.  Two waves per SIMD at least (the accumulators
-// then live in VGPRs; FP32 reaches three).
+// reads of a group overlap.
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm(
-    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles, T* __restrict__ arena,
-    const double* __restrict__ dpiv) {
+__device__ __forceinline__ void gemm_tile(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                          T* __restrict__ arena, const double* __restrict__ dpiv) {
   using M = Mfma<T>;
   using V4 = typename M::acc_t;
   using L16 = typename M::lds16_t;
@@ -131,6 +129,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         if (col < t.cend) F[row + static_cast<int64_t>(col) * m] = cv[bj][r] - acc[bi][bj][r];
       }
   }
+}
+
+// The same tile under two register budgets: k_gemm asks for two waves per
+// SIMD (accumulators in VGPRs; FP32 reaches three), k_gemm_agpr leaves the
+// accumulators in AGPRs (two waves per SIMD in FP32, one in FP64).  Which is
+// faster depends on the launch (tile count, K, front sizes); the solver times
+// both on each launch of its plan once and keeps the faster.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm(
+    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles, T* __restrict__ arena,
+    const double* __restrict__ dpiv) {
+  gemm_tile<T>(fd, tiles, arena, dpiv);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_agpr(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                                   T* __restrict__ arena, const double* __restrict__ dpiv) {
+  gemm_tile<T>(fd, tiles, arena, dpiv);
 }
 
 }  // namespace rf

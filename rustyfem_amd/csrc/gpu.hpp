@@ -18,6 +18,9 @@ struct Timings {
   double rel_residual = -1;
   // per-launch HIP-event timing of the Schur-update GEMM kernel (when enabled)
   double gemm_ms = 0, gemm_launches = 0, gemm_flops = 0;
+  // GEMM autotuning at the first factorization: its time, the GEMM launches
+  // of the plan and how many of them run the AGPR-accumulator variant
+  double tune_ms = 0, tuned_launches = 0, agpr_launches = 0;
 };
 
 class GpuFactor {

@@ -1359,6 +1359,12 @@ class Impl final : public GpuFactor {
       legacy_panel_ = q && *q && *q != '0';
       const char* c = std::getenv("RUSTYFEM_SOLVE_CHAIN");
       chain_solve_ = c && *c && *c != '0';
+      // RUSTYFEM_AUTOTUNE: unset or 1 = time both GEMM kernels per launch,
+      // 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere
+      const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
+      const bool force_agpr = a && std::strcmp(a, "agpr") == 0;
+      autotune_ = !(a && *a == '0') && !force_agpr;
+      gemm_pick_.assign(P_.gemm.size(), force_agpr ? 1 : 0);
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -1427,6 +1433,7 @@ class Impl final : public GpuFactor {
     HIPCHK(hipEventRecord(e0_, s_));
     launch(k_gather, (n_ + 255) / 256, 256, d_dpos_, d_lval_, d_adiag_, n_);
     launch(k_gather, (nfull_ + 255) / 256, 256, d_gmap_, d_lval_, d_av_, nfull_);
+    if (autotune_ && !tuned_) tune_gemm();
     nev_ = 0;
     tm_.gemm_flops = 0;
     for (const LevelPlan& lp : P_.levels) {
@@ -1564,12 +1571,69 @@ class Impl final : public GpuFactor {
       HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
     }
     if (ph.kind != PhaseKind::Gemm128) throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
-    launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    if (gemm_pick_[r.begin])
+      launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else
+      launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
       ++nev_;
     }
+  }
+
+  // Autotuning of the GEMM register layout: every GEMM launch of the plan runs
+  // both variants (one warm-up, then three timed launches each, the minimum
+  // kept) and keeps k_gemm_agpr where it is at least 3% faster.  The launches
+  // work on the arena before the first factorization assembles anything in
+  // it; every level zeroes and assembles its fronts before its own GEMMs, so
+  // what they leave there is never read.
+  void tune_gemm() {
+    std::vector<const Phase*> ph;
+    for (const LevelPlan& lp : P_.levels)
+      for (const Phase& p : lp.phases)
+        if (p.kind == PhaseKind::Gemm128 && p.r.count > 0) ph.push_back(&p);
+    std::vector<hipEvent_t> ev(ph.size() * 8);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(e0_, s_));
+    for (size_t i = 0; i < ph.size(); ++i) {
+      const Range& r = ph[i]->r;
+      for (int v = 0; v < 2; ++v) {
+        auto run = [&]() {
+          if (v)
+            launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+          else
+            launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+        };
+        run();
+        for (int k = 0; k < 4; ++k) {
+          HIPCHK(hipEventRecord(ev[8 * i + 4 * v + k], s_));
+          if (k < 3) run();
+        }
+      }
+    }
+    HIPCHK(hipEventRecord(e1_, s_));
+    HIPCHK(hipStreamSynchronize(s_));
+    int64_t alt = 0;
+    for (size_t i = 0; i < ph.size(); ++i) {
+      float best[2] = {1e30f, 1e30f};
+      for (int v = 0; v < 2; ++v)
+        for (int k = 0; k < 3; ++k) {
+          float t = 0;
+          HIPCHK(hipEventElapsedTime(&t, ev[8 * i + 4 * v + k], ev[8 * i + 4 * v + k + 1]));
+          best[v] = std::min(best[v], t);
+        }
+      const bool pick = best[1] < 0.97f * best[0];
+      gemm_pick_[ph[i]->r.begin] = pick ? 1 : 0;
+      alt += pick ? 1 : 0;
+    }
+    for (auto& e : ev) HIPCHK(hipEventDestroy(e));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
+    tm_.tune_ms = ms;
+    tm_.tuned_launches = static_cast<double>(ph.size());
+    tm_.agpr_launches = static_cast<double>(alt);
+    tuned_ = true;
   }
 
   void ensure_vectors(int nrhs) {
@@ -1854,6 +1918,8 @@ class Impl final : public GpuFactor {
   double* d_segsq_ = nullptr;
   int nseg_ = 0, segsq_cap_ = 0;
   bool legacy_panel_ = false;
+  bool autotune_ = true, tuned_ = false;
+  std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): 1 = k_gemm_agpr
   bool chain_solve_ = false;
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;

@@ -255,10 +255,11 @@ class Handle:
         return o
 
     def stats(self):
-        o = np.zeros(13)
-        lib().rustyfem_stats(self._h, o.ctypes.data, 13)
+        o = np.zeros(16)
+        lib().rustyfem_stats(self._h, o.ctypes.data, 16)
         keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
-                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops"]
+                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
+                "tuned_launches", "agpr_launches"]
         return dict(zip(keys, o.tolist()))
 
 
@@ -378,10 +379,11 @@ class Batch:
         return {"problems": int(o[0]), "n": int(o[1]), "fronts": int(o[2]), "levels": int(o[3]), "work": o[4]}
 
     def stats(self):
-        o = np.zeros(13)
-        lib().rustyfem_batch_stats(self._h, o.ctypes.data, 13)
+        o = np.zeros(16)
+        lib().rustyfem_batch_stats(self._h, o.ctypes.data, 16)
         keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
-                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops"]
+                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
+                "tuned_launches", "agpr_launches"]
         return dict(zip(keys, o.tolist()))
 
     def order(self, i):
