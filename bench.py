@@ -178,6 +178,78 @@ def run_amg(args):
     print(json.dumps(out), flush=True)
 
 
+def run_learned(args):
+    """C3D4 block (Kuhn split of tet_size^3 cubes, perturbed interior vertices,
+    lognormal stiffness per cube with a log-std drawn from U[0.5, 2], seed 42,
+    clamped base, body load): AMG-FCG whose finest smoother is the learned one
+    (parameters from rustyfem_amd/data/smoother_c3d4.npz) vs AMG-PCG with
+    Chebyshev smoothing, both FP64, to ||r|| <= 1e-8 ||b||.  One step = one
+    solve from x0 = 0 with the preconditioner set up and b in HBM."""
+    import numpy as np
+    from rustyfem_amd import ffi, hip, meshes
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rustyfem_amd", "data", "smoother_c3d4.npz")
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"].copy()
+        meta = json.loads(str(z["meta"]))
+    ffi.lib()
+    t0 = time.perf_counter()
+    p = meshes.tet_block(args.tet_size, seed=42)
+    _, coords, ndof, _, _ = meshes.node_graph(p)
+    cp, ri, v = lower_of(p)
+    log(f"problem tet {args.tet_size}^3 cubes: n={p.n} log-std {p.meta['log_std']:.3f} built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    b = np.ascontiguousarray(p.rhs[:, 0])
+    d_b = hip.DeviceArray.from_host(b)
+    tol = 1e-8
+    res = {}
+    cases = (("amg_pcg_chebyshev", {"krylov": "pcg"}, None),
+             ("learned_fcg", {"krylov": "fcg"}, params))
+    for name, kw, prm in cases:
+        t0 = time.perf_counter()
+        s = ffi.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, **kw)
+        if prm is not None:
+            s.set_learned_smoother(prm, sweeps=2)
+        st = s.stats()
+        log(f"{name}: setup {time.perf_counter() - t0:.1f}s levels {st['sizes']}")
+        d_x = hip.DeviceArray(p.n)
+        for _ in range(args.warmup):
+            s.solve_device(d_b.ptr, d_x.ptr, tol=tol)
+        times = []
+        for i in range(args.steps):
+            hip.synchronize()
+            t1 = time.perf_counter()
+            it, rel, conv = s.solve_device(d_b.ptr, d_x.ptr, tol=tol)
+            hip.synchronize()
+            times.append(time.perf_counter() - t1)
+            log(f"{name}: step {i} {times[-1] * 1e3:.1f} ms iterations {it} rel {rel:.2e} converged {conv}")
+        x = d_x.download()
+        true_rel = float(np.linalg.norm(b - p.to_scipy() @ x) / np.linalg.norm(b))
+        st = s.stats()
+        res[name] = {"time_s": float(np.median(times)), "iterations": it, "converged": bool(conv),
+                     "rel_residual": rel, "true_rel_residual": true_rel, "setup_s": st["setup_ms"] * 1e-3,
+                     "levels": st["sizes"], "spmv_ms": st["spmv_ms"], "spmv_bytes": st["spmv_bytes"], "solution": x}
+        d_x.free()
+        del s
+    a, f = res["learned_fcg"], res["amg_pcg_chebyshev"]
+    achieved = a["spmv_bytes"] / (a["spmv_ms"] * 1e-3) / 1e9
+    err = float(np.max(np.abs(a["solution"] - f["solution"])) / np.max(np.abs(f["solution"])))
+    for r in res.values():
+        r.pop("solution")
+    out = {"metric": "AMG-Krylov solve time to ||r||/||b|| <= 1e-8, learned-smoother FCG vs Chebyshev AMG-PCG",
+           "value": a["time_s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": a["time_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": f"C3D4 block, {args.tet_size}^3 cubes x 6 tetrahedra (n={p.n}), perturbed interior "
+                                  f"vertices, lognormal E (log-std {p.meta['log_std']:.3f}), clamped base, body load",
+                      "n": p.n, "smoother_training": meta},
+           "learned_fcg": a, "amg_pcg_chebyshev": f, "speedup_learned_vs_chebyshev": f["time_s"] / a["time_s"],
+           "learned_vs_chebyshev_displacement_rel_error": err,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                        "frac": achieved / PEAK["hbm_gbs"], "traffic": None,
+                        "kernel": "k_spmv<double,3,G> (BSR 3x3 fine-matrix product, FP64)"}}
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """512 independent CPE4 plates (nx ~ U{20..200}, ny = nx/10, E ~ U[70e9, 210e9],
     nu ~ U[0.25, 0.35], 8 edge-traction load cases each, seed 3), factored and
@@ -269,9 +341,11 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="cube edge of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["direct", "amg", "batch"], default="direct",
+    ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned"], default="direct",
                     help="direct: the multifrontal block (default); amg: the heterogeneous block with AMG-Krylov; "
-                         "batch: 512 plates factored and solved together")
+                         "batch: 512 plates factored and solved together; learned: the C3D4 block with the "
+                         "learned-smoother FCG")
+    ap.add_argument("--tet-size", type=int, default=96, help="cubes per edge of the C3D4 block (learned workload)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied device-to-device twice before the run (a kernel of known traffic for "
@@ -286,6 +360,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "learned":
+        run_learned(args)
         return
     os.environ.setdefault("RUSTYFEM_KERNEL_TIMING", "1")
     import numpy as np

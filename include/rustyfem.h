@@ -167,7 +167,9 @@ typedef struct PcgHandle PcgHandle;
  * IterMatrix (src/solver/iterative.rs:208-243).  kind: 0 AMG (smoothed
  * aggregation with rigid body modes, src/solver/iterative/amg.rs:282-341;
  * the hierarchy is built on the host, the spectral radii on the GPU),
- * 2 Jacobi.  NULL on error (rustyfem_last_error). */
+ * 1 IC(0) (src/solver/iterative/csr.rs:219-324: factored and applied on the
+ * GPU level by level, with the diagonal-shift retry of csr.rs:250-256),
+ * 2 Jacobi (csr.rs:198-217).  NULL on error (rustyfem_last_error). */
 PcgHandle *rustyfem_pcg_setup(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,
                               int32_t kind, size_t nn, const double *coords, const int64_t *ndof,
                               const rustyfem_pcg_options *opt);
@@ -181,9 +183,43 @@ int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol,
 
 /* [0] setup ms, [1] last solve ms, [2] levels, [3] operator complexity,
  * [4] ms of one fine-matrix product, [5] its bytes, [6] coarsest size,
- * [7] iterations of the last solve, [8..19] rows per level. */
+ * [7] iterations of the last solve, [8..19] rows per level, [20] IC(0)
+ * diagonal shift (Ic0::shift, src/solver/iterative/csr.rs:228), [21] IC(0)
+ * forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts. */
 void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
+
+/* z = M r: one application of the handle's preconditioner (Precond::apply,
+ * src/solver/iterative/csr.rs:193-196; for IC(0) csr.rs:296-316, for Jacobi
+ * csr.rs:209-215).  r and z are host pointers, or device pointers when
+ * on_device != 0.  0 on success, -1 on error. */
+int32_t rustyfem_pcg_precondition(PcgHandle *h, const double *r, double *z, int32_t on_device);
 void rustyfem_pcg_free(PcgHandle *h);
+
+/* Learned smoother (additive, no counterpart in the reference): the AMG
+ * hierarchy's finest smoother becomes x = M b, then x += M (b - A x), `sweeps`
+ * steps before and after the coarse correction, with M computed on the GPU by
+ * the message-passing network of the learned-smoother parameters (nparams =
+ * rustyfem_mpnn_nparams()).  AMG handles with 3 DOFs per node only; use with
+ * krylov = 1 (flexible CG).  0 on success, -1 on error. */
+int32_t rustyfem_pcg_set_learned_smoother(PcgHandle *h, const double *params, size_t nparams, int32_t sweeps);
+
+/* Training interface of the learned smoother (additive).  The network maps a
+ * 3x3-block stiffness matrix (lower triangle in CSC, natural node-major
+ * numbering, 3 DOFs per node) to a symmetric block matrix M on its block
+ * pattern; the layout of the parameters is documented in csrc/mpnn.hpp. */
+typedef struct MpnnHandle MpnnHandle;
+int64_t rustyfem_mpnn_nparams(void);
+MpnnHandle *rustyfem_mpnn_create(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val);
+/* L = mean over the nvec columns of e0 (n x nvec, column-major) of
+ * e_nu^T A e_nu / e_0^T A e_0 with e_{k+1} = (I - M A) e_k; grad (nparams,
+ * may be NULL) receives dL/dparams.  NaN on error. */
+double rustyfem_mpnn_loss(MpnnHandle *h, const double *params, const double *e0, size_t nvec, int32_t nu,
+                          double *grad);
+/* M's blocks (nnzb x 9, row-major 3x3) and block pattern (n/3 + 1 row
+ * pointers, nnzb sorted block columns); any output may be NULL.  Returns nnzb
+ * (-1 on error). */
+int64_t rustyfem_mpnn_smoother(MpnnHandle *h, const double *params, double *blocks, int64_t *ptr, int32_t *idx);
+void rustyfem_mpnn_free(MpnnHandle *h);
 
 /* One-shot setup + solve with the default options: mirrors
  * solve_lower(k, b, kind, coords, ndof, tol) -> (x, PcgReport)

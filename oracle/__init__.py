@@ -81,6 +81,8 @@ def lib():
             "ora_solve_lower": (C.c_int64, [sz, i64p, i64p, f64p, f64p, C.c_int, sz, f64p, i64p, C.c_double,
                                             C.c_int64, f64p, C.POINTER(C.c_double), C.POINTER(C.c_int32), vp,
                                             C.POINTER(C.c_double)]),
+            "ora_ic0_factor": (C.c_int64, [sz, i64p, i64p, f64p, vp, vp, vp, vp, C.POINTER(C.c_double)]),
+            "ora_precond_apply": (None, [sz, i64p, i64p, f64p, C.c_int, f64p, f64p]),
             "ora_natural_pattern": (sz, [sz, i64p, i64p, i64p, vp, vp, vp]),
             "ora_sym_matvec": (None, [sz, i64p, i64p, f64p, f64p, f64p]),
             "ora_rigid_body_modes": (sz, [sz, f64p, i64p, f64p]),
@@ -350,6 +352,29 @@ def solve_lower(n, colptr, rowidx, val, b, kind="amg", coords=None, ndof=None, t
                                C.byref(cx))
     return x, {"iterations": int(it), "rel_residual": rel.value, "converged": bool(conv.value),
                "amg_sizes": [int(s) for s in sizes if s], "complexity": cx.value}
+
+
+def ic0_factor(n, colptr, rowidx, val):
+    """IC(0) factor (src/solver/iterative/csr.rs:229-257): (ptr, idx, lval, d,
+    shift) with the strictly lower L row-wise."""
+    colptr, rowidx, val = _i64(colptr), _i64(rowidx), _f64(val)
+    nnz = int(lib().ora_ic0_factor(n, colptr, rowidx, val, None, None, None, None, None))
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    idx = np.zeros(max(nnz, 1), dtype=np.int64)
+    lval = np.zeros(max(nnz, 1))
+    d = np.zeros(max(n, 1))
+    shift = C.c_double(0.0)
+    lib().ora_ic0_factor(n, colptr, rowidx, val, ptr.ctypes.data, idx.ctypes.data, lval.ctypes.data, d.ctypes.data,
+                         C.byref(shift))
+    return ptr, idx[:nnz], lval[:nnz], d[:n], shift.value
+
+
+def precond_apply(n, colptr, rowidx, val, r, kind="ic0"):
+    """z = M r (Precond::apply, src/solver/iterative/csr.rs:193-217, 296-316)."""
+    k = {"ic0": 1, "jacobi": 2}[kind]
+    z = np.zeros(n)
+    lib().ora_precond_apply(n, _i64(colptr), _i64(rowidx), _f64(val), k, _f64(r), z)
+    return z
 
 
 def natural_pattern(ndof, adj_ptr, adj_idx):

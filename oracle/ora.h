@@ -121,6 +121,13 @@ int64_t ora_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, 
                         const double *b, int kind, size_t nn, const double *coords, const int64_t *ndof,
                         double tol, int64_t maxit, double *x, double *rel, int32_t *converged,
                         int64_t *amg_sizes /* 16 or NULL */, double *complexity);
+/* IC(0) factor (src/solver/iterative/csr.rs:229-257): strictly lower L
+ * row-wise and D, shift of the last attempt; returns nnz(L) (outputs may be NULL) */
+int64_t ora_ic0_factor(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, int64_t *ptr,
+                       int64_t *idx, double *lval, double *d, double *shift);
+/* z = M r, kind 1 IC(0) or 2 Jacobi (src/solver/iterative/csr.rs:193-217, 296-316) */
+void ora_precond_apply(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, int kind,
+                       const double *r, double *z);
 /* natural pattern (src/solver/iterative.rs:74-97): returns nnz; first (nn) */
 size_t ora_natural_pattern(size_t nn, const int64_t *adj_ptr, const int64_t *adj_idx, const int64_t *ndof,
                            int64_t *first, int64_t *colptr, int64_t *rowidx);

@@ -922,6 +922,50 @@ int64_t ora_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, 
   return it;
 }
 
+/* IC(0) factor of the lower CSC matrix (src/solver/iterative/csr.rs:229-257):
+ * strictly lower L row-wise (ptr n+1, idx/val nnz(L)) and D; returns nnz(L).
+ * With NULL outputs only the count is returned. */
+int64_t ora_ic0_factor(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, int64_t *ptr,
+                       int64_t *idx, double *lval, double *d, double *shift) {
+  csr a = csr_from_lower(n, colptr, rowidx, val);
+  precond pc;
+  memset(&pc, 0, sizeof(pc));
+  pc.kind = 1;
+  ic0_new(&pc, &a);
+  int64_t nnz = pc.ptr[n];
+  if (ptr) memcpy(ptr, pc.ptr, (n + 1) * sizeof(int64_t));
+  if (idx) memcpy(idx, pc.idx, (size_t)nnz * sizeof(int64_t));
+  if (lval) memcpy(lval, pc.lval, (size_t)nnz * sizeof(double));
+  if (d) memcpy(d, pc.d, n * sizeof(double));
+  if (shift) *shift = pc.shift;
+  precond_free(&pc);
+  csr_free(&a);
+  return nnz;
+}
+
+/* z = M r for the IC(0) (kind 1) or Jacobi (kind 2) preconditioner of the
+ * lower CSC matrix (Precond::apply, src/solver/iterative/csr.rs:193-217,
+ * 296-316). */
+void ora_precond_apply(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, int kind,
+                       const double *r, double *z) {
+  csr a = csr_from_lower(n, colptr, rowidx, val);
+  precond pc;
+  memset(&pc, 0, sizeof(pc));
+  pc.kind = kind;
+  if (kind == 2) {
+    double *dg = csr_diag(&a);
+    pc.inv = (double *)ora_xmalloc((n + 1) * sizeof(double));
+    for (size_t i = 0; i < n; i++) pc.inv[i] = dg[i] != 0.0 ? 1.0 / dg[i] : 1.0;
+    free(dg);
+  } else {
+    pc.kind = 1;
+    ic0_new(&pc, &a);
+  }
+  precond_apply(&pc, r, z, n);
+  precond_free(&pc);
+  csr_free(&a);
+}
+
 /* src/solver/iterative.rs:74-97 */
 size_t ora_natural_pattern(size_t nn, const int64_t *adj_ptr, const int64_t *adj_idx, const int64_t *ndof,
                            int64_t *first, int64_t *colptr, int64_t *rowidx) {

@@ -19,6 +19,7 @@
 #include "gpu.hpp"
 #include "indef.hpp"
 #include "mf.hpp"
+#include "mpnn_train.hpp"
 #include "pcg.hpp"
 #include "symbolic.hpp"
 
@@ -693,6 +694,87 @@ int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol,
 }
 
 void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) { h->solver->stats(out, len); }
+
+int32_t rustyfem_pcg_precondition(PcgHandle* h, const double* r, double* z, int32_t on_device) {
+  try {
+    std::lock_guard<std::mutex> lock(h->mu);
+    h->solver->apply(r, z, on_device != 0);
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int32_t rustyfem_pcg_set_learned_smoother(PcgHandle* h, const double* params, size_t nparams, int32_t sweeps) {
+  try {
+    if (static_cast<int64_t>(nparams) != rf::mpnn_nparams()) throw std::runtime_error("wrong number of parameters");
+    std::lock_guard<std::mutex> lock(h->mu);
+    h->solver->set_learned_smoother(params, sweeps);
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+struct MpnnHandle {
+  std::unique_ptr<rf::MpnnTrainer> t;
+  int64_t n = 0;
+};
+
+int64_t rustyfem_mpnn_nparams(void) { return rf::mpnn_nparams(); }
+
+MpnnHandle* rustyfem_mpnn_create(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val) {
+  set_error("");
+  try {
+    for (size_t c = 0; c < n; ++c) {
+      if (colptr[c + 1] < colptr[c]) throw std::runtime_error("colptr is not monotone");
+      for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p)
+        if (rowidx[p] < static_cast<int64_t>(c) || rowidx[p] >= static_cast<int64_t>(n))
+          throw std::runtime_error("rowidx outside the lower triangle");
+    }
+    auto h = std::make_unique<MpnnHandle>();
+    h->n = static_cast<int64_t>(n);
+    h->t = rf::make_mpnn_trainer(static_cast<int64_t>(n), colptr, rowidx, val);
+    return h.release();
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return nullptr;
+}
+
+double rustyfem_mpnn_loss(MpnnHandle* h, const double* params, const double* e0, size_t nvec, int32_t nu,
+                          double* grad) {
+  try {
+    return h->t->loss(params, e0, static_cast<int>(nvec), nu, grad);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return std::nan("");
+}
+
+int64_t rustyfem_mpnn_smoother(MpnnHandle* h, const double* params, double* blocks, int64_t* ptr, int32_t* idx) {
+  try {
+    h->t->smoother(params, blocks, ptr, idx);
+    return h->t->nnzb();
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_mpnn_free(MpnnHandle* h) { delete h; }
 
 void rustyfem_pcg_free(PcgHandle* h) { delete h; }
 

@@ -1,7 +1,7 @@
 // GPU preconditioned Krylov solver for the lower-triangle matrices of the
 // iterative path (src/solver/iterative.rs:275-292, csr.rs:326-380): AMG
-// V-cycle (Chebyshev or damped-Jacobi smoothing, FP64 or FP32), Jacobi;
-// standard PCG or flexible CG.
+// V-cycle (Chebyshev or damped-Jacobi smoothing, FP64 or FP32), IC(0),
+// Jacobi; standard PCG or flexible CG.
 #pragma once
 #include <cstdint>
 #include <memory>
@@ -32,12 +32,21 @@ class PcgGpu {
   // [0] setup ms (host + GPU), [1] last solve ms, [2] levels (incl. coarsest),
   // [3] operator complexity, [4] fine-matrix SpMV ms (one product, measured),
   // [5] fine-matrix bytes per SpMV, [6] coarsest size, [7] iterations of the last solve,
-  // [8..19] rows per level (finest first)
+  // [8..19] rows per level (finest first), [20] IC(0) diagonal shift,
+  // [21] IC(0) forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts
   virtual void stats(double* out, size_t len) const = 0;
+  // z = M r, one application of the preconditioner (Precond::apply,
+  // src/solver/iterative/csr.rs:193-196); r and z in host or device memory
+  virtual void apply(const double* r, double* z, bool on_device) = 0;
   virtual std::vector<int64_t> level_sizes() const = 0;
+  // AMG only, 3 DOFs per node: the finest level's smoother becomes the learned
+  // one of mpnn.hpp (x = M b, then x += M (b - A x), `sweeps` steps before and
+  // after the coarse correction), M computed on the GPU from the finest
+  // matrix and the network parameters (MPNN_NPARAMS doubles, host)
+  virtual void set_learned_smoother(const double* params, int sweeps) = 0;
 };
 
-// kind: 0 AMG (smoothed aggregation, rigid body modes), 2 Jacobi.  coords /
+// kind: 0 AMG (smoothed aggregation, rigid body modes), 1 IC(0), 2 Jacobi.  coords /
 // ndof describe the node blocks (natural numbering: node v owns ndof[v]
 // consecutive equations).  Throws when the AMG setup breaks down.
 std::unique_ptr<PcgGpu> make_pcg_gpu(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,

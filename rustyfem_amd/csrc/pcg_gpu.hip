@@ -27,6 +27,8 @@
 #include <type_traits>
 
 #include "amg.hpp"
+#include "ic0.hpp"
+#include "mpnn.hpp"
 #include "pcg.hpp"
 
 namespace rf {
@@ -287,6 +289,11 @@ __global__ __launch_bounds__(RB) void k_update_p(int64_t n, double* p, const dou
   const double b = sc->beta;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * RB)
     p[i] = z[i] + b * p[i];
+}
+
+__global__ __launch_bounds__(RB) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x; t < n; t += static_cast<int64_t>(gridDim.x) * RB)
+    b[t] = static_cast<float>(a[t]);
 }
 
 __global__ __launch_bounds__(RB) void k_jacobi(int64_t n, const double* __restrict__ dinv, const double* __restrict__ r,
@@ -576,7 +583,7 @@ class PcgImpl final : public PcgGpu {
   }
   PcgResult solve(const double* b, double* x, double tol, int64_t maxit, bool on_device) override;
   void stats(double* out, size_t len) const override {
-    std::vector<double> v(20, 0.0);
+    std::vector<double> v(24, 0.0);
     v[0] = setup_ms_;
     v[1] = solve_ms_;
     v[2] = static_cast<double>(sizes_.size());
@@ -586,9 +593,17 @@ class PcgImpl final : public PcgGpu {
     v[6] = sizes_.empty() ? 0.0 : static_cast<double>(sizes_.back());
     v[7] = static_cast<double>(last_it_);
     for (size_t i = 0; i < sizes_.size() && i < 12; ++i) v[8 + i] = static_cast<double>(sizes_[i]);
+    if (ic0_) {
+      v[20] = ic0_->shift();
+      v[21] = static_cast<double>(ic0_->forward_levels());
+      v[22] = static_cast<double>(ic0_->backward_levels());
+      v[23] = ic0_->attempts();
+    }
     for (size_t i = 0; i < len && i < v.size(); ++i) out[i] = v[i];
   }
   std::vector<int64_t> level_sizes() const override { return sizes_; }
+  void set_learned_smoother(const double* params, int sweeps) override;
+  void apply(const double* r, double* z, bool on_device) override;
 
  private:
   template <typename T>
@@ -609,9 +624,12 @@ class PcgImpl final : public PcgGpu {
   std::vector<VLevel<float>> v32_;
   DBuf cm64_, cx64_, cb64_, cm32_, cx32_, cb32_;
   DBuf jdinv_;
+  std::unique_ptr<Ic0Gpu> ic0_;
   DBuf x_, r_, z_, p_, q_, part_, scal_;
   DBuf b32_, x32_;
   hipGraphExec_t graph_ = nullptr;
+  DevMat ml_;              // learned finest-level smoother (A's pattern), when set
+  int learned_sweeps_ = 0;
   std::vector<int64_t> sizes_;
   double complexity_ = 1.0, setup_ms_ = 0, solve_ms_ = 0, spmv_ms_ = 0;
   int64_t last_it_ = 0;
@@ -700,7 +718,7 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
     : n_(n), kind_(kind), opt_(opt) {
   const auto t0 = std::chrono::steady_clock::now();
   PHIPCHK(hipStreamCreate(&stream_));
-  if (kind != 0 && kind != 2) throw std::runtime_error("preconditioner kind not available on the GPU (0 AMG, 2 Jacobi)");
+  if (kind < 0 || kind > 2) throw std::runtime_error("preconditioner kind must be 0 (AMG), 1 (IC(0)) or 2 (Jacobi)");
   Csr a = csr_from_lower(n, colptr, rowidx, val);
   // block size of the finest matrix: the common DOF count of the nodes
   int bs = nn > 0 ? static_cast<int>(ndof[0]) : 1;
@@ -717,6 +735,8 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
     std::vector<double> d = csr_diag(a);
     for (double& v : d) v = v != 0.0 ? 1.0 / v : 1.0;
     jdinv_ = upload(d);
+  } else if (kind == 1) {
+    ic0_ = make_ic0_gpu(a, stream_);
   } else {
     std::vector<int64_t> bptr(nn + 1, 0);
     for (int64_t v = 0; v < nn; ++v) bptr[v + 1] = bptr[v] + ndof[v];
@@ -766,11 +786,20 @@ void PcgImpl::vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done) 
   T* r = V.r.template as<T>();
   T* d = V.d.template as<T>();
   const T* dinv = V.dinv.template as<T>();
+  const bool learned = l == 0 && learned_sweeps_ > 0;
   // pre-smoothing from x = 0
-  k_smooth0<T><<<g, RB, 0, s>>>(V.n, dinv, b, x, d, V.c0, done);
-  for (const auto& st : V.steps) {
-    spmv<T>(*V.A, 1, x, r, b, done, s);
-    k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, st.first, st.second, done);
+  if (learned) {
+    spmv<T>(ml_, 0, b, x, nullptr, done, s);
+    for (int k = 1; k < learned_sweeps_; ++k) {
+      spmv<T>(*V.A, 1, x, r, b, done, s);
+      spmv<T>(ml_, 2, r, x, nullptr, done, s);
+    }
+  } else {
+    k_smooth0<T><<<g, RB, 0, s>>>(V.n, dinv, b, x, d, V.c0, done);
+    for (const auto& st : V.steps) {
+      spmv<T>(*V.A, 1, x, r, b, done, s);
+      k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, st.first, st.second, done);
+    }
   }
   // coarse-grid correction
   spmv<T>(*V.A, 1, x, r, b, done, s);
@@ -786,6 +815,13 @@ void PcgImpl::vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done) 
   }
   spmv<T>(V.P, 2, xc, x, nullptr, done, s);
   // post-smoothing
+  if (learned) {
+    for (int k = 0; k < learned_sweeps_; ++k) {
+      spmv<T>(*V.A, 1, x, r, b, done, s);
+      spmv<T>(ml_, 2, r, x, nullptr, done, s);
+    }
+    return;
+  }
   spmv<T>(*V.A, 1, x, r, b, done, s);
   k_smooth<T><<<g, RB, 0, s>>>(V.n, dinv, r, x, d, T(0), V.c0, done);
   for (const auto& st : V.steps) {
@@ -794,10 +830,49 @@ void PcgImpl::vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done) 
   }
 }
 
+void PcgImpl::set_learned_smoother(const double* params, int sweeps) {
+  if (kind_ != 0) throw std::runtime_error("learned smoother: AMG preconditioner required");
+  if (A_.bs != 3) throw std::runtime_error("learned smoother: 3 DOFs per node required");
+  if (sweeps < 1) throw std::runtime_error("learned smoother: sweeps must be positive");
+  DBuf prm = upload(std::vector<double>(params, params + MPNN_NPARAMS));
+  DevMat m;
+  m.nbr = A_.nbr;
+  m.nbc = A_.nbc;
+  m.bs = 3;
+  m.group = A_.group;
+  m.nnzb = A_.nnzb;
+  m.ptr.alloc((A_.nbr + 1) * sizeof(int64_t));
+  m.idx.alloc(A_.nnzb * sizeof(int32_t));
+  m.v64.alloc(A_.nnzb * 9 * sizeof(double));
+  PHIPCHK(hipMemcpyAsync(m.ptr.p, A_.ptr.p, (A_.nbr + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, stream_));
+  PHIPCHK(hipMemcpyAsync(m.idx.p, A_.idx.p, A_.nnzb * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+  {
+    MpnnGraph net(A_.nbr, A_.nnzb, A_.ptr.as<int64_t>(), A_.idx.as<int32_t>(), A_.v64.as<double>(), stream_);
+    net.forward(prm.as<double>(), m.v64.as<double>());
+    PHIPCHK(hipStreamSynchronize(stream_));
+  }
+  if (opt_.vcycle_fp32) {
+    m.v32.alloc(A_.nnzb * 9 * sizeof(float));
+    k_to_f32<<<grid_stride(A_.nnzb * 9), RB, 0, stream_>>>(A_.nnzb * 9, m.v64.as<double>(), m.v32.as<float>());
+    PHIPCHK(hipGetLastError());
+    PHIPCHK(hipStreamSynchronize(stream_));
+  }
+  ml_ = std::move(m);
+  learned_sweeps_ = sweeps;
+  if (graph_) {
+    PHIPCHK(hipGraphExecDestroy(graph_));
+    graph_ = nullptr;
+  }
+}
+
 void PcgImpl::precondition(const double* r, double* z, const int32_t* done) {
   const Scal* sc = scal_.as<Scal>();
   if (kind_ == 2) {
     k_jacobi<<<grid_stride(n_), RB, 0, stream_>>>(n_, jdinv_.as<double>(), r, z, sc);
+    return;
+  }
+  if (kind_ == 1) {
+    ic0_->apply(r, z, done);
     return;
   }
   const unsigned g = grid_stride(n_);
@@ -816,6 +891,18 @@ void PcgImpl::precondition(const double* r, double* z, const int32_t* done) {
     else vcycle<double>(v64_, 0, done);
     k_copy<<<g, RB, 0, stream_>>>(n_, x0, z, sc);
   }
+}
+
+void PcgImpl::apply(const double* r, double* z, bool on_device) {
+  if (n_ == 0) return;
+  // a cleared state: the kernels skip their work once the done flag is set
+  Scal s0{};
+  PHIPCHK(hipMemcpyAsync(scal_.p, &s0, sizeof s0, hipMemcpyHostToDevice, stream_));
+  PHIPCHK(hipMemcpyAsync(r_.p, r, n_ * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  precondition(r_.as<double>(), z_.as<double>(), &scal_.as<Scal>()->done);
+  PHIPCHK(hipGetLastError());
+  PHIPCHK(hipMemcpyAsync(z, z_.p, n_ * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
+  PHIPCHK(hipStreamSynchronize(stream_));
 }
 
 // one Krylov iteration (csr.rs:356-377)

@@ -23,7 +23,9 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats", "rustyfem_pcg_setup",
            "rustyfem_pcg_solve", "rustyfem_pcg_stats", "rustyfem_pcg_free", "rustyfem_solve_lower",
            "rustyfem_factor_batch", "rustyfem_refactor_batch", "rustyfem_solve_batch", "rustyfem_solve_batch_device",
-           "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch"]
+           "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch",
+           "rustyfem_pcg_set_learned_smoother", "rustyfem_mpnn_nparams", "rustyfem_mpnn_create", "rustyfem_mpnn_loss",
+           "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition"]
 
 _lib = None
 
@@ -89,8 +91,22 @@ def lib():
         L.rustyfem_pcg_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int64, C.c_int32, f64p, C.POINTER(C.c_int32)]
         L.rustyfem_pcg_stats.restype = None
         L.rustyfem_pcg_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_pcg_precondition.restype = C.c_int32
+        L.rustyfem_pcg_precondition.argtypes = [vp, vp, vp, C.c_int32]
         L.rustyfem_pcg_free.restype = None
         L.rustyfem_pcg_free.argtypes = [vp]
+        L.rustyfem_pcg_set_learned_smoother.restype = C.c_int32
+        L.rustyfem_pcg_set_learned_smoother.argtypes = [vp, vp, sz, C.c_int32]
+        L.rustyfem_mpnn_nparams.restype = C.c_int64
+        L.rustyfem_mpnn_nparams.argtypes = []
+        L.rustyfem_mpnn_create.restype = vp
+        L.rustyfem_mpnn_create.argtypes = [sz, vp, vp, vp]
+        L.rustyfem_mpnn_loss.restype = C.c_double
+        L.rustyfem_mpnn_loss.argtypes = [vp, vp, vp, sz, C.c_int32, vp]
+        L.rustyfem_mpnn_smoother.restype = C.c_int64
+        L.rustyfem_mpnn_smoother.argtypes = [vp, vp, vp, vp, vp]
+        L.rustyfem_mpnn_free.restype = None
+        L.rustyfem_mpnn_free.argtypes = [vp]
         pp = C.POINTER(C.c_void_p)
         L.rustyfem_factor_batch.restype = vp
         L.rustyfem_factor_batch.argtypes = [sz, vp, vp, pp, pp, pp, pp, pp, C.POINTER(Options), vp]
@@ -309,16 +325,78 @@ class Pcg:
         return int(it), float(rel.value), bool(conv.value)
 
     def stats(self):
-        o = np.zeros(20)
-        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 20)
+        o = np.zeros(24)
+        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 24)
         lv = int(o[2])
         return {"setup_ms": o[0], "solve_ms": o[1], "levels": lv, "complexity": o[3], "spmv_ms": o[4],
                 "spmv_bytes": o[5], "coarse": int(o[6]), "iterations": int(o[7]),
-                "sizes": [int(v) for v in o[8:8 + min(lv, 12)]]}
+                "sizes": [int(v) for v in o[8:8 + min(lv, 12)]], "ic0_shift": o[20],
+                "ic0_forward_levels": int(o[21]), "ic0_backward_levels": int(o[22]), "ic0_attempts": int(o[23])}
+
+    def precondition(self, r):
+        """z = M r, one application of the preconditioner."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.n)
+        if lib().rustyfem_pcg_precondition(self._h, r.ctypes.data, z.ctypes.data, 0) != 0:
+            raise RuntimeError(last_error())
+        return z
+
+    def set_learned_smoother(self, params, sweeps=2):
+        """Finest-level smoother x += M (b - A x) with M from the learned network."""
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        if lib().rustyfem_pcg_set_learned_smoother(self._h, params.ctypes.data, params.size, int(sweeps)) != 0:
+            raise RuntimeError(last_error())
 
     def __del__(self):
         if getattr(self, "_h", None):
             lib().rustyfem_pcg_free(self._h)
+            self._h = None
+
+
+def mpnn_nparams():
+    return int(lib().rustyfem_mpnn_nparams())
+
+
+class Mpnn:
+    """Training handle of the learned smoother for one matrix (lower-triangle CSC,
+    natural node-major numbering, 3 DOFs per node)."""
+
+    def __init__(self, n, colptr, rowidx, val):
+        self.n = int(n)
+        self._keep = [np.ascontiguousarray(colptr, np.int64), np.ascontiguousarray(rowidx, np.int64),
+                      np.ascontiguousarray(val, np.float64)]
+        self._h = lib().rustyfem_mpnn_create(self.n, *[a.ctypes.data for a in self._keep])
+        self._keep = None
+        if not self._h:
+            raise RuntimeError(last_error())
+
+    def loss(self, params, e0, nu=2, grad=False):
+        """Smoothing loss for the columns of e0 (n x nvec); with grad=True also dL/dparams."""
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        e0 = np.asfortranarray(e0, dtype=np.float64).reshape(self.n, -1, order="F")
+        g = np.zeros(params.size) if grad else None
+        flat = np.ascontiguousarray(e0.reshape(-1, order="F"))
+        val = lib().rustyfem_mpnn_loss(self._h, params.ctypes.data, flat.ctypes.data, e0.shape[1], int(nu),
+                                       g.ctypes.data if grad else None)
+        if not np.isfinite(val):
+            raise RuntimeError(last_error())
+        return (val, g) if grad else val
+
+    def smoother(self, params):
+        """(ptr, idx, blocks (nnzb, 3, 3)) of M."""
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        nnzb = lib().rustyfem_mpnn_smoother(self._h, params.ctypes.data, None, None, None)
+        if nnzb < 0:
+            raise RuntimeError(last_error())
+        blocks = np.zeros(nnzb * 9)
+        ptr = np.zeros(self.n // 3 + 1, dtype=np.int64)
+        idx = np.zeros(nnzb, dtype=np.int32)
+        lib().rustyfem_mpnn_smoother(self._h, params.ctypes.data, blocks.ctypes.data, ptr.ctypes.data, idx.ctypes.data)
+        return ptr, idx, blocks.reshape(nnzb, 3, 3)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().rustyfem_mpnn_free(self._h)
             self._h = None
 
 

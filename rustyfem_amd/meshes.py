@@ -471,9 +471,11 @@ def tet4_stiffness(x: np.ndarray, e: np.ndarray, nu: float):
 
 
 def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4, perturb=0.2, chunk=400000,
-              clamp="z0"):
+              clamp="z0", min_quality=0.1):
     """C3D4 block on nx^3 cubes of the unit cube, each split into six tetrahedra
-    (KUHN), interior vertices moved by U[-perturb h, perturb h]^3, a lognormal
+    (KUHN), interior vertices moved by U[-perturb h, perturb h]^3 (the moves of
+    the vertices of a tetrahedron left with less than min_quality of its
+    undistorted volume are halved until none is left), a lognormal
     Young's modulus per cube (log-std drawn from U[0.5, 2] when not given,
     correlation length corr_len), clamped base z = 0 (clamp=None: no support),
     body load along -z.
@@ -487,7 +489,8 @@ def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4
     kk, jj, ii = np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij")
     coords = np.stack([ii.ravel() * h, jj.ravel() * h, kk.ravel() * h], axis=1)
     inner = ((ii > 0) & (ii < nx) & (jj > 0) & (jj < nx) & (kk > 0) & (kk < nx)).ravel()
-    coords[inner] += rng.uniform(-perturb * h, perturb * h, size=(int(inner.sum()), 3))
+    disp = np.zeros_like(coords)
+    disp[inner] = rng.uniform(-perturb * h, perturb * h, size=(int(inner.sum()), 3))
     field = lognormal_field((nx, nx, nx), log_std, corr_len, int(rng.integers(1 << 31)))
     # corner c of cube (i, j, k) is node (i + c&1) + n1 (j + (c>>1)&1) + n1^2 (k + (c>>2)&1)
     ck, cj, ci = np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx), indexing="ij")
@@ -500,6 +503,22 @@ def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4
         if np.linalg.det(cube[tets[t, 1:]] - cube[tets[t, 0]]) < 0:
             tets[t, [2, 3]] = tets[t, [3, 2]]
     conn = (base[:, None, None] + corner[tets][None, :, :]).reshape(-1, 4)   # (6 nx^3, 4), cube-major
+    # every tetrahedron keeps at least min_quality of its undistorted volume:
+    # the vertices of one that does not are pulled halfway back, until none is left
+    for _ in range(40):
+        x = coords + disp
+        bad = []
+        for s0 in range(0, conn.shape[0], chunk):
+            v = x[conn[s0:s0 + chunk]]
+            det = np.linalg.det(v[:, 1:] - v[:, :1])
+            bad.append(s0 + np.nonzero(det < min_quality * h ** 3)[0])
+        bad = np.concatenate(bad)
+        if bad.size == 0:
+            break
+        disp[np.unique(conn[bad])] *= 0.5
+    else:
+        raise ValueError("tet_block: distorted tetrahedra remain")
+    coords = coords + disp
     ecube = np.repeat(np.arange(nx ** 3), 6)
     estiff = e * field.ravel()[ecube]
     # stencil offsets sorted by node index: d in {-1,0,1}^3 with all components of one sign

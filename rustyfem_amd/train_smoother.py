@@ -1,0 +1,146 @@
+"""Trains the learned smoother (csrc/mpnn.hpp) on C3D4 tetrahedral blocks.
+
+    python -m rustyfem_amd.train_smoother --systems 2000 --nx 32 --epochs 1 --out rustyfem_amd/data/smoother_c3d4.npz
+
+Every system is a ``meshes.tet_block(nx, seed=s)`` (s = first_seed, first_seed
++ 1, ...: perturbed tetrahedra, lognormal stiffness with a log-std drawn per
+system from U[0.5, 2]), generated in worker processes and streamed.  Adam
+steps (steps_per_system per system and epoch) minimize the smoothing loss of
+nvec random Gaussian error vectors,
+mean ||(I - M A)^nu e||_A^2 / ||e||_A^2, computed with its gradient on the GPU
+(rustyfem_mpnn_loss).  The network sees dimensionless block features
+(s_i s_j A_ij), so systems of a few thousand unknowns train it for any mesh
+size of the same element family.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+NPARAMS_LAYOUT = (16 * 10, 16, 16 * 48, 16, 9 * 16, 9)
+
+
+def lower_of(p):
+    import scipy.sparse as sp
+    a = p.to_scipy().tocoo()
+    k = a.row >= a.col
+    low = sp.csc_matrix((a.data[k], (a.row[k], a.col[k])), shape=a.shape)
+    low.sort_indices()
+    return low.indptr.astype(np.int64), low.indices.astype(np.int64), low.data.copy()
+
+
+def initial_params(seed=0, scale=0.1, out_scale=1e-3):
+    """Random hidden layers and a small output layer: the smoother starts at
+    scaled damped Jacobi."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal(NPARAMS_LAYOUT[0]) * scale, rng.standard_normal(NPARAMS_LAYOUT[1]) * scale,
+             rng.standard_normal(NPARAMS_LAYOUT[2]) * scale, rng.standard_normal(NPARAMS_LAYOUT[3]) * scale,
+             rng.standard_normal(NPARAMS_LAYOUT[4]) * out_scale, np.zeros(NPARAMS_LAYOUT[5])]
+    return np.concatenate(parts)
+
+
+def _system(args):
+    """(n, lower CSC) of tet_block(nx, seed) (run in a worker process)."""
+    nx, s = args
+    from rustyfem_amd import meshes
+    p = meshes.tet_block(nx, seed=s)
+    return p.n, lower_of(p)
+
+
+def train(systems=256, nx=10, first_seed=1000, epochs=4, nvec=8, nu=2, lr=3e-3, seed=0, workers=12,
+          steps_per_system=2, log=print):
+    """Adam over the systems, streamed from a process pool: every system is
+    generated once per epoch and gets steps_per_system steps with fresh error
+    vectors."""
+    import multiprocessing as mp
+    # the worker processes start before this process touches the GPU
+    pool = mp.get_context("spawn").Pool(max(1, workers))
+    from rustyfem_amd import ffi
+    ffi.lib()
+    assert sum(NPARAMS_LAYOUT) == ffi.mpnn_nparams()
+    rng = np.random.default_rng(seed)
+    w = initial_params(seed)
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    step = 0
+    hist = []
+    t0 = time.perf_counter()
+    with pool:
+        for ep in range(epochs):
+            losses = []
+            order = [(nx, int(s)) for s in first_seed + rng.permutation(systems)]
+            for k, (n, low) in enumerate(pool.imap(_system, order, chunksize=1)):
+                h = ffi.Mpnn(n, *low)
+                for _ in range(steps_per_system):
+                    e0 = rng.standard_normal((n, nvec))
+                    loss, g = h.loss(w, e0, nu=nu, grad=True)
+                    gn = np.linalg.norm(g)
+                    if gn > 1.0:
+                        g = g / gn
+                    step += 1
+                    m = b1 * m + (1 - b1) * g
+                    v = b2 * v + (1 - b2) * g * g
+                    w = w - lr * (m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps)
+                    losses.append(loss)
+                del h
+                if (k + 1) % 100 == 0:
+                    log(f"epoch {ep}: {k + 1} systems, mean loss of the last 100 {np.mean(losses[-100 * steps_per_system:]):.5f} "
+                        f"({time.perf_counter() - t0:.1f}s)")
+            hist.append(float(np.mean(losses)))
+            log(f"epoch {ep}: mean smoothing loss {hist[-1]:.5f} ({time.perf_counter() - t0:.1f}s)")
+    return w, hist
+
+
+def evaluate(params, systems=16, nx=10, first_seed=900, nvec=8, nu=2, seed=1):
+    """Mean smoothing loss of params and of the starting point on held-out systems."""
+    from rustyfem_amd import ffi, meshes
+    rng = np.random.default_rng(seed)
+    start = initial_params(0)
+    start[-(9 * 16 + 9):] = 0.0
+    lt, l0 = [], []
+    for s in range(first_seed, first_seed + systems):
+        p = meshes.tet_block(nx, seed=s)
+        h = ffi.Mpnn(p.n, *lower_of(p))
+        e0 = rng.standard_normal((h.n, nvec))
+        lt.append(h.loss(params, e0, nu=nu))
+        l0.append(h.loss(start, e0, nu=nu))
+    return float(np.mean(lt)), float(np.mean(l0))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--systems", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=10)
+    ap.add_argument("--first-seed", type=int, default=1000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--nvec", type=int, default=8)
+    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--workers", type=int, default=12, help="processes generating systems")
+    ap.add_argument("--steps-per-system", type=int, default=2)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "smoother_c3d4.npz"))
+    args = ap.parse_args()
+
+    def log(msg):
+        print(f"[train] {msg}", file=sys.stderr, flush=True)
+
+    w, hist = train(args.systems, args.nx, args.first_seed, args.epochs, args.nvec, args.nu, args.lr,
+                    workers=args.workers, steps_per_system=args.steps_per_system, log=log)
+    lt, l0 = evaluate(w, nx=args.nx, nvec=args.nvec, nu=args.nu)
+    log(f"held-out systems: trained loss {lt:.5f}, scaled damped Jacobi {l0:.5f}")
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    np.savez(args.out, params=w, history=np.array(hist),
+             meta=json.dumps({"systems": args.systems, "nx": args.nx, "first_seed": args.first_seed,
+                              "epochs": args.epochs, "nvec": args.nvec, "nu": args.nu, "lr": args.lr,
+                              "heldout_loss": lt, "heldout_jacobi_loss": l0}))
+    print(json.dumps({"heldout_loss": lt, "heldout_jacobi_loss": l0, "history": hist}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

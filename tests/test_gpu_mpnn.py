@@ -1,0 +1,92 @@
+"""GPU learned smoother (csrc/mpnn_gpu.hip) against the CPU restatement
+oracle/mpnn.py: the forward pass (M's blocks), the smoothing loss, the
+parameter gradient against central finite differences of the oracle's loss,
+and the smoother inside AMG flexible CG (no counterpart in the reference;
+pinned by the true residual and the direct solution)."""
+import numpy as np
+import pytest
+
+from oracle import mpnn
+from rustyfem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rf():
+    from rustyfem_amd import ffi
+    ffi.lib()
+    return ffi
+
+
+def lower_of(p):
+    import scipy.sparse as sp
+    a = p.to_scipy().tocoo()
+    k = a.row >= a.col
+    low = sp.csc_matrix((a.data[k], (a.row[k], a.col[k])), shape=a.shape)
+    low.sort_indices()
+    return low.indptr.astype(np.int64), low.indices.astype(np.int64), low.data.copy()
+
+
+def test_forward_matches_oracle(rf):
+    p = meshes.tet_block(4, seed=2)
+    cp, ri, v = lower_of(p)
+    print(f"case mpnn forward tet4 n={p.n}", flush=True)
+    prm = mpnn.init_params(1, out_scale=0.3)
+    h = rf.Mpnn(p.n, cp, ri, v)
+    ptr, idx, blocks = h.smoother(prm)
+    optr, oidx, oblk = mpnn.bsr_from_lower(p.n, cp, ri, v)
+    assert np.array_equal(ptr, optr) and np.array_equal(idx, oidx)
+    om = mpnn.smoother(prm, optr, oidx, oblk)
+    assert np.max(np.abs(blocks - om)) <= 1e-12 * np.max(np.abs(om))
+
+
+def test_loss_and_gradient_match_oracle(rf):
+    p = meshes.tet_block(3, seed=7)
+    cp, ri, v = lower_of(p)
+    print(f"case mpnn gradient tet3 n={p.n}", flush=True)
+    prm = mpnn.init_params(2, out_scale=0.2)
+    e0 = np.random.default_rng(3).standard_normal((p.n, 4))
+    h = rf.Mpnn(p.n, cp, ri, v)
+    loss, g = h.loss(prm, e0, nu=2, grad=True)
+    assert h.loss(prm, e0, nu=2) == loss
+    optr, oidx, oblk = mpnn.bsr_from_lower(p.n, cp, ri, v)
+    assert np.isclose(loss, mpnn.loss(prm, optr, oidx, oblk, e0, nu=2), rtol=1e-12)
+    # central differences of the oracle's loss, four parameters of every group
+    sizes = [mpnn.H * mpnn.F, mpnn.H, mpnn.H * 3 * mpnn.H, mpnn.H, 9 * mpnn.H, 9]
+    starts = np.cumsum([0] + sizes[:-1])
+    rng = np.random.default_rng(5)
+    picks = np.concatenate([st + rng.choice(sz, 4, replace=False) for st, sz in zip(starts, sizes)])
+    gmax = np.max(np.abs(g))
+    assert gmax > 0
+    for k in picks:
+        d = 1e-6 * max(1.0, abs(prm[k]))
+        hi, lo = prm.copy(), prm.copy()
+        hi[k] += d
+        lo[k] -= d
+        fd = (mpnn.loss(hi, optr, oidx, oblk, e0, nu=2) - mpnn.loss(lo, optr, oidx, oblk, e0, nu=2)) / (2 * d)
+        assert abs(fd - g[k]) <= 1e-5 * gmax, (k, fd, g[k])
+
+
+def test_learned_smoother_in_fcg(rf, ora):
+    """The network's starting point (zero output layer: scaled damped Jacobi)
+    and a random small output layer as finest smoother of AMG-FCG."""
+    p = meshes.tet_block(10, seed=9)
+    cp, ri, v = lower_of(p)
+    print(f"case mpnn fcg tet10 n={p.n}", flush=True)
+    from rustyfem_amd.meshes import node_graph
+    _, coords, ndof, _, _ = node_graph(p)
+    b = p.rhs[:, 0].copy()
+    a = p.to_scipy()
+    hd, _, _ = ora.Handle.factor(p)
+    xd = hd.solve(b)
+    for prm in (np.concatenate([mpnn.init_params(4)[:-(9 * mpnn.H + 9)], np.zeros(9 * mpnn.H + 9)]),
+                mpnn.init_params(6, out_scale=0.02)):
+        s = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg")
+        assert s.stats()["levels"] >= 2
+        s.set_learned_smoother(prm, sweeps=2)
+        x, it, rel, conv = s.solve(b, tol=1e-8)
+        assert conv and it > 1
+        assert np.linalg.norm(a @ x - b) / np.linalg.norm(b) < 1e-7
+        assert np.max(np.abs(x - xd)) / np.max(np.abs(xd)) < 1e-5
+        print(f"iterations {it}", flush=True)
