@@ -181,10 +181,13 @@ def run_amg(args):
 def run_learned(args):
     """C3D4 block (Kuhn split of tet_size^3 cubes, perturbed interior vertices,
     lognormal stiffness per cube with a log-std drawn from U[0.5, 2], seed 42,
-    clamped base, body load): AMG-FCG whose finest smoother is the learned one
-    (parameters from rustyfem_amd/data/smoother_c3d4.npz) vs AMG-PCG with
-    Chebyshev smoothing, both FP64, to ||r|| <= 1e-8 ||b||.  One step = one
-    solve from x0 = 0 with the preconditioner set up and b in HBM."""
+    clamped base, body load), to ||r|| <= 1e-8 ||b||, three solvers on the same
+    AMG hierarchy: plain AMG-PCG (FP64 V-cycle, Chebyshev smoothing; the
+    baseline and the fallback), AMG-FCG with the FP32 V-cycle and Chebyshev
+    smoothing (the method without the network), and the same FCG whose finest
+    smoother is the learned one (parameters from
+    rustyfem_amd/data/smoother_c3d4.npz, one sweep).  One step = one solve
+    from x0 = 0 with the preconditioner set up and b in HBM."""
     import numpy as np
     from rustyfem_amd import ffi, hip, meshes
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rustyfem_amd", "data", "smoother_c3d4.npz")
@@ -201,16 +204,11 @@ def run_learned(args):
     b = np.ascontiguousarray(p.rhs[:, 0])
     d_b = hip.DeviceArray.from_host(b)
     tol = 1e-8
+    a_mat = p.to_scipy()
     res = {}
-    cases = (("amg_pcg_chebyshev", {"krylov": "pcg"}, None),
-             ("learned_fcg", {"krylov": "fcg"}, params))
-    for name, kw, prm in cases:
-        t0 = time.perf_counter()
-        s = ffi.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, **kw)
-        if prm is not None:
-            s.set_learned_smoother(prm, sweeps=2)
+
+    def timed(name, s):
         st = s.stats()
-        log(f"{name}: setup {time.perf_counter() - t0:.1f}s levels {st['sizes']}")
         d_x = hip.DeviceArray(p.n)
         for _ in range(args.warmup):
             s.solve_device(d_b.ptr, d_x.ptr, tol=tol)
@@ -223,27 +221,43 @@ def run_learned(args):
             times.append(time.perf_counter() - t1)
             log(f"{name}: step {i} {times[-1] * 1e3:.1f} ms iterations {it} rel {rel:.2e} converged {conv}")
         x = d_x.download()
-        true_rel = float(np.linalg.norm(b - p.to_scipy() @ x) / np.linalg.norm(b))
+        d_x.free()
         st = s.stats()
         res[name] = {"time_s": float(np.median(times)), "iterations": it, "converged": bool(conv),
-                     "rel_residual": rel, "true_rel_residual": true_rel, "setup_s": st["setup_ms"] * 1e-3,
-                     "levels": st["sizes"], "spmv_ms": st["spmv_ms"], "spmv_bytes": st["spmv_bytes"], "solution": x}
-        d_x.free()
-        del s
-    a, f = res["learned_fcg"], res["amg_pcg_chebyshev"]
+                     "rel_residual": rel, "true_rel_residual": float(np.linalg.norm(b - a_mat @ x) / np.linalg.norm(b)),
+                     "setup_s": st["setup_ms"] * 1e-3, "levels": st["sizes"], "spmv_ms": st["spmv_ms"],
+                     "spmv_bytes": st["spmv_bytes"], "fallback": st["fallback"], "solution": x}
+
+    t0 = time.perf_counter()
+    s = ffi.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="pcg", vcycle="fp64")
+    log(f"AMG setup (FP64 V-cycle) {time.perf_counter() - t0:.1f}s levels {s.stats()['sizes']}")
+    timed("amg_pcg_fp64", s)
+    del s
+    t0 = time.perf_counter()
+    s = ffi.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg", vcycle="fp32")
+    log(f"AMG setup (FP32 V-cycle) {time.perf_counter() - t0:.1f}s")
+    timed("fcg_fp32_chebyshev", s)
+    t0 = time.perf_counter()
+    s.set_learned_smoother(params, sweeps=1)
+    log(f"learned smoother M computed in {time.perf_counter() - t0:.2f}s")
+    timed("learned_fcg_fp32", s)
+    del s
+    a, f, c = res["learned_fcg_fp32"], res["amg_pcg_fp64"], res["fcg_fp32_chebyshev"]
     achieved = a["spmv_bytes"] / (a["spmv_ms"] * 1e-3) / 1e9
     err = float(np.max(np.abs(a["solution"] - f["solution"])) / np.max(np.abs(f["solution"])))
     for r in res.values():
         r.pop("solution")
-    out = {"metric": "AMG-Krylov solve time to ||r||/||b|| <= 1e-8, learned-smoother FCG vs Chebyshev AMG-PCG",
+    out = {"metric": "AMG-Krylov solve time to ||r||/||b|| <= 1e-8, learned-smoother FCG vs plain AMG-PCG",
            "value": a["time_s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": a["time_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-           "dtype": "fp64", "data": "synthetic",
+           "dtype": "fp64+fp32", "data": "synthetic",
            "config": {"workload": f"C3D4 block, {args.tet_size}^3 cubes x 6 tetrahedra (n={p.n}), perturbed interior "
                                   f"vertices, lognormal E (log-std {p.meta['log_std']:.3f}), clamped base, body load",
                       "n": p.n, "smoother_training": meta},
-           "learned_fcg": a, "amg_pcg_chebyshev": f, "speedup_learned_vs_chebyshev": f["time_s"] / a["time_s"],
-           "learned_vs_chebyshev_displacement_rel_error": err,
+           "learned_fcg_fp32": a, "fcg_fp32_chebyshev": c, "amg_pcg_fp64": f,
+           "speedup_vs_plain_amg_pcg": f["time_s"] / a["time_s"],
+           "gain_of_network_over_chebyshev": c["time_s"] / a["time_s"],
+           "learned_vs_plain_displacement_rel_error": err,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                         "frac": achieved / PEAK["hbm_gbs"], "traffic": None,
                         "kernel": "k_spmv<double,3,G> (BSR 3x3 fine-matrix product, FP64)"}}

@@ -185,7 +185,13 @@ int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol,
  * [4] ms of one fine-matrix product, [5] its bytes, [6] coarsest size,
  * [7] iterations of the last solve, [8..19] rows per level, [20] IC(0)
  * diagonal shift (Ic0::shift, src/solver/iterative/csr.rs:228), [21] IC(0)
- * forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts. */
+ * forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts,
+ * [24] 1 when the last solve fell back from the learned smoother to plain
+ * AMG-PCG, [25] its true relative residual ||b - Ax|| / ||b|| (learned
+ * smoother set; else -1), [26] iterations of the abandoned learned-smoother
+ * attempt, [27] learned-smoother sweeps (0 = none), [28..39] DOFs per
+ * block of the level matrices, finest first (6 x 6 blocks for the aggregates
+ * of a 3D hierarchy; 1 = scalar CSR). */
 void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
 
 /* z = M r: one application of the handle's preconditioner (Precond::apply,
@@ -200,7 +206,11 @@ void rustyfem_pcg_free(PcgHandle *h);
  * steps before and after the coarse correction, with M computed on the GPU by
  * the message-passing network of the learned-smoother parameters (nparams =
  * rustyfem_mpnn_nparams()).  AMG handles with 3 DOFs per node only; use with
- * krylov = 1 (flexible CG).  0 on success, -1 on error. */
+ * krylov = 1 (flexible CG).  Plain AMG-PCG is the fallback and the judge:
+ * each solve checks the true FP64 residual of its result, and a breakdown,
+ * a residual that has not halved over 40 iterations or a true residual above
+ * tol restarts the solve with the Chebyshev V-cycle and standard PCG
+ * (rustyfem_pcg_stats out[24..26]).  0 on success, -1 on error. */
 int32_t rustyfem_pcg_set_learned_smoother(PcgHandle *h, const double *params, size_t nparams, int32_t sweeps);
 
 /* Training interface of the learned smoother (additive).  The network maps a

@@ -46,6 +46,7 @@ struct AmgHostLevel {
 struct AmgHost {
   std::vector<AmgHostLevel> levels;   // smoothed levels (finest first)
   Csr coarse;                         // the coarsest matrix
+  std::vector<int64_t> coarse_bptr;   // its DOF blocks (one per aggregate of the last level)
   std::vector<double> coarse_l, coarse_d;  // its dense LDL^T (unit L row-major, D)
   std::vector<int64_t> sizes;         // rows per level, finest first, coarsest last
   double complexity = 0.0;            // total nnz / fine nnz

@@ -444,6 +444,7 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
     b = std::move(bc);
     theta *= 0.5;
   }
+  H.coarse_bptr = bptr;
   // dense LDL^T of the coarsest matrix (amg.rs:319-340)
   const int64_t n = a.nrows;
   std::vector<double> l(static_cast<size_t>(n) * n, 0.0), d(n, 0.0);

@@ -45,16 +45,19 @@ namespace {
 
 // ---------------------------------------------------------------- assembly
 
+// Clears the whole arena (every front's m x m block) once per factorization:
+// one grid-stride stream of 16-byte stores, HBM-bound.  The arena is not
+// reused across levels, so nothing written later is cleared by it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_zero(const FrontDesc* __restrict__ fd, const ZeroTile* __restrict__ tiles,
-                                              T* __restrict__ arena) {
-  const ZeroTile t = tiles[blockIdx.x];
-  const FrontDesc d = fd[t.front];
-  const int cend = min(t.c0 + ZC, d.m);
-  for (int c = t.c0; c < cend; ++c) {
-    T* col = arena + d.off + static_cast<int64_t>(c) * d.m;
-    for (int r = c + static_cast<int>(threadIdx.x); r < d.m; r += 256) col[r] = T(0);
-  }
+__global__ __launch_bounds__(256) void k_clear(T* __restrict__ arena, int64_t n) {
+  const int64_t nv = n * static_cast<int64_t>(sizeof(T)) / 16;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  float4* a4 = reinterpret_cast<float4*>(arena);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < nv; i += stride)
+    a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t i = nv * 16 / static_cast<int64_t>(sizeof(T)) + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+       i < n; i += stride)
+    arena[i] = T(0);
 }
 
 template <typename T>
@@ -1332,7 +1335,6 @@ class Impl final : public GpuFactor {
     d_fd_ = dupload(P_.fd, bytes_, s_);
     d_cmap_ = dupload(P_.cmap, bytes_, s_);
     d_amap_ = dupload(P_.amap, bytes_, s_);
-    d_zero_ = dupload(P_.zero, bytes_, s_);
     d_scatter_ = dupload(P_.scatter, bytes_, s_);
     d_ea_ = dupload(P_.ea, bytes_, s_);
     d_gemm_ = dupload(P_.gemm, bytes_, s_);
@@ -1415,7 +1417,7 @@ class Impl final : public GpuFactor {
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
     drop_graphs();
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
@@ -1436,8 +1438,12 @@ class Impl final : public GpuFactor {
     if (autotune_ && !tuned_) tune_gemm();
     nev_ = 0;
     tm_.gemm_flops = 0;
+    {
+      const int64_t nv = P_.arena_elems * static_cast<int64_t>(sizeof(T)) / 16;
+      launch(k_clear<T>, std::max<int64_t>(1, std::min<int64_t>(8192, (nv + 255) / 256)), 256, d_arena_,
+             P_.arena_elems);
+    }
     for (const LevelPlan& lp : P_.levels) {
-      launch(k_zero<T>, lp.zero.count, 256, d_fd_, d_zero_ + lp.zero.begin, d_arena_);
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
@@ -1586,7 +1592,8 @@ class Impl final : public GpuFactor {
   // both variants (one warm-up, then three timed launches each, the minimum
   // kept) and keeps k_gemm_agpr where it is at least 3% faster.  The launches
   // work on the arena before the first factorization assembles anything in
-  // it; every level zeroes and assembles its fronts before its own GEMMs, so
+  // it; the factorization clears the arena and every level assembles its
+  // fronts before its own GEMMs, so
   // what they leave there is never read.
   void tune_gemm() {
     std::vector<const Phase*> ph;
@@ -1898,7 +1905,6 @@ class Impl final : public GpuFactor {
   FrontDesc* d_fd_ = nullptr;
   int32_t* d_cmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
-  ZeroTile* d_zero_ = nullptr;
   ScatterTile* d_scatter_ = nullptr;
   EaTile* d_ea_ = nullptr;
   GemmTile* d_gemm_ = nullptr;

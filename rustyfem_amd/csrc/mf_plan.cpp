@@ -108,11 +108,6 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     int64_t b = static_cast<int64_t>(P.level_fronts.size());
     P.level_fronts.insert(P.level_fronts.end(), lp.fronts.begin(), lp.fronts.end());
     lp.fronts_r = take(b, static_cast<int64_t>(P.level_fronts.size()));
-    // zero tiles
-    b = static_cast<int64_t>(P.zero.size());
-    for (int32_t f : lp.fronts)
-      for (int32_t c0 = 0; c0 < P.fd[f].m; c0 += ZC) P.zero.push_back({f, c0});
-    lp.zero = take(b, static_cast<int64_t>(P.zero.size()));
     // scatter tiles over the Lower entries of the level's pivot columns
     b = static_cast<int64_t>(P.scatter.size());
     for (int32_t f : lp.fronts) {

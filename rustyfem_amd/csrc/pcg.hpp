@@ -33,7 +33,11 @@ class PcgGpu {
   // [3] operator complexity, [4] fine-matrix SpMV ms (one product, measured),
   // [5] fine-matrix bytes per SpMV, [6] coarsest size, [7] iterations of the last solve,
   // [8..19] rows per level (finest first), [20] IC(0) diagonal shift,
-  // [21] IC(0) forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts
+  // [21] IC(0) forward levels, [22] IC(0) backward levels, [23] IC(0) factor attempts,
+  // [24] 1 when the last solve fell back from the learned smoother to plain AMG-PCG,
+  // [25] true relative residual of the last solve (learned smoother set; else -1),
+  // [26] iterations of the abandoned learned-smoother attempt, [27] learned sweeps,
+  // [28..39] DOFs per block of the level matrices (finest first; 1 = scalar CSR)
   virtual void stats(double* out, size_t len) const = 0;
   // z = M r, one application of the preconditioner (Precond::apply,
   // src/solver/iterative/csr.rs:193-196); r and z in host or device memory
@@ -42,7 +46,11 @@ class PcgGpu {
   // AMG only, 3 DOFs per node: the finest level's smoother becomes the learned
   // one of mpnn.hpp (x = M b, then x += M (b - A x), `sweeps` steps before and
   // after the coarse correction), M computed on the GPU from the finest
-  // matrix and the network parameters (MPNN_NPARAMS doubles, host)
+  // matrix and the network parameters (MPNN_NPARAMS doubles, host).  Every
+  // solve then checks the true FP64 residual of its result; a breakdown, a
+  // residual that has not halved over 40 iterations, or a true residual above
+  // the tolerance makes the solve start again with plain AMG-PCG (Chebyshev
+  // smoothing, standard PCG), whose result is returned.
   virtual void set_learned_smoother(const double* params, int sweeps) = 0;
 };
 

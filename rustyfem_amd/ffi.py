@@ -325,13 +325,15 @@ class Pcg:
         return int(it), float(rel.value), bool(conv.value)
 
     def stats(self):
-        o = np.zeros(24)
-        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 24)
+        o = np.zeros(40)
+        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 40)
         lv = int(o[2])
         return {"setup_ms": o[0], "solve_ms": o[1], "levels": lv, "complexity": o[3], "spmv_ms": o[4],
                 "spmv_bytes": o[5], "coarse": int(o[6]), "iterations": int(o[7]),
                 "sizes": [int(v) for v in o[8:8 + min(lv, 12)]], "ic0_shift": o[20],
-                "ic0_forward_levels": int(o[21]), "ic0_backward_levels": int(o[22]), "ic0_attempts": int(o[23])}
+                "ic0_forward_levels": int(o[21]), "ic0_backward_levels": int(o[22]), "ic0_attempts": int(o[23]),
+                "fallback": bool(o[24]), "true_rel_residual": o[25], "abandoned_iterations": int(o[26]),
+                "learned_sweeps": int(o[27]), "blocks": [int(b) for b in o[28:28 + min(lv, 12)]]}
 
     def precondition(self, r):
         """z = M r, one application of the preconditioner."""
@@ -341,7 +343,7 @@ class Pcg:
             raise RuntimeError(last_error())
         return z
 
-    def set_learned_smoother(self, params, sweeps=2):
+    def set_learned_smoother(self, params, sweeps=1):
         """Finest-level smoother x += M (b - A x) with M from the learned network."""
         params = np.ascontiguousarray(params, dtype=np.float64)
         if lib().rustyfem_pcg_set_learned_smoother(self._h, params.ctypes.data, params.size, int(sweeps)) != 0:

@@ -90,3 +90,76 @@ def test_learned_smoother_in_fcg(rf, ora):
         assert np.linalg.norm(a @ x - b) / np.linalg.norm(b) < 1e-7
         assert np.max(np.abs(x - xd)) / np.max(np.abs(xd)) < 1e-5
         print(f"iterations {it}", flush=True)
+        st = s.stats()
+        assert not st["fallback"] and st["learned_sweeps"] == 2
+        assert 0 < st["true_rel_residual"] <= 1e-8
+
+
+def _trained():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "..", "rustyfem_amd", "data", "smoother_c3d4.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].copy()
+
+
+def _untrained(prm):
+    """The same hidden layers with a zero output layer: scaled damped Jacobi."""
+    u = prm.copy()
+    u[-(9 * mpnn.H + 9):] = 0.0
+    return u
+
+
+def test_trained_network_gains_over_its_starting_point(rf):
+    """What training buys on held-out C3D4 blocks (seeds outside the training
+    range 1000-2999): a smaller smoothing loss, and fewer AMG-FCG iterations
+    than the same network before training (its zero output layer is scaled
+    damped Jacobi)."""
+    prm = _trained()
+    p = meshes.tet_block(12, seed=77)
+    cp, ri, v = lower_of(p)
+    print(f"case mpnn trained vs untrained tet12 n={p.n}", flush=True)
+    h = rf.Mpnn(p.n, cp, ri, v)
+    e0 = np.random.default_rng(8).standard_normal((p.n, 8))
+    lt, l0 = h.loss(prm, e0, nu=2), h.loss(_untrained(prm), e0, nu=2)
+    print(f"smoothing loss trained {lt:.5f} untrained {l0:.5f}", flush=True)
+    assert lt < 0.25 * l0
+    from rustyfem_amd.meshes import node_graph
+    _, coords, ndof, _, _ = node_graph(p)
+    b = p.rhs[:, 0].copy()
+    its = {}
+    for name, w in (("trained", prm), ("untrained", _untrained(prm)), ("chebyshev", None)):
+        s = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg")
+        if w is not None:
+            s.set_learned_smoother(w, sweeps=2)
+        x, it, rel, conv = s.solve(b, tol=1e-8)
+        assert conv and rel <= 1e-8
+        if w is not None:
+            assert not s.stats()["fallback"]
+        its[name] = it
+    print(f"FCG iterations {its}", flush=True)
+    assert its["trained"] < its["untrained"]
+
+
+def test_fallback_to_plain_amg_pcg(rf):
+    """A learned smoother that breaks the Krylov recursion (non-finite network
+    output, forced from the host) is caught and the solve is redone with plain
+    AMG-PCG; the true residual is the judge."""
+    p = meshes.tet_block(8, seed=3)
+    cp, ri, v = lower_of(p)
+    print(f"case mpnn fallback tet8 n={p.n}", flush=True)
+    from rustyfem_amd.meshes import node_graph
+    _, coords, ndof, _, _ = node_graph(p)
+    b = p.rhs[:, 0].copy()
+    plain = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof)
+    xp, itp, _, _ = plain.solve(b, tol=1e-8)
+    bad = _trained()
+    bad[-9:] = np.nan
+    s = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg")
+    s.set_learned_smoother(bad, sweeps=2)
+    for _ in range(2):
+        x, it, rel, conv = s.solve(b, tol=1e-8)
+        st = s.stats()
+        assert conv and st["fallback"] and st["true_rel_residual"] <= 1e-8
+        assert it == itp and np.array_equal(x, xp)
+        a = p.to_scipy()
+        assert np.linalg.norm(a @ x - b) / np.linalg.norm(b) <= 1e-8 * (1 + 1e-6)

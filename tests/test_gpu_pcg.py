@@ -62,6 +62,8 @@ def test_amg_pcg_matches_direct_and_oracle_levels(rf, ora, block12):
     assert st["levels"] == len(rep["amg_sizes"])
     assert abs(st["complexity"] - rep["complexity"]) < 1e-6 * rep["complexity"]
     print(f"iterations gpu={it} oracle={rep['iterations']} sizes={st['sizes']}", flush=True)
+    # the transfer operators in 3 x 6 / 6 x 3 blocks (node DOFs x rigid body modes)
+    assert st["blocks"][:2] == [3, 6]
 
 
 def test_fcg_reproduces_pcg(rf, block12):
@@ -123,6 +125,43 @@ def test_plane_plate_three_modes(rf, ora):
     h, _, _ = ora.Handle.factor(p)
     xd = h.solve(b)
     assert np.max(np.abs(x - xd)) / np.max(np.abs(xd)) < 1e-7
+
+
+def test_plane_plate_block_levels(rf, ora):
+    """A CPE4 plate large enough for a hierarchy: the transfer operators in
+    2 x 3 / 3 x 2 blocks (node DOFs x planar rigid body modes)."""
+    p = meshes.plate_cpe4(60, 60, 210e9, 0.3, nrhs=1)
+    print(f"case plate blocks n={p.n}", flush=True)
+    s, _ = setup(rf, p)
+    st = s.stats()
+    assert st["levels"] >= 2 and st["blocks"][:2] == [2, 3]
+    b = p.rhs[:, 0].copy()
+    x, it, rel, conv = s.solve(b, tol=1e-10)
+    assert conv
+    h, _, _ = ora.Handle.factor(p)
+    xd = h.solve(b)
+    assert np.max(np.abs(x - xd)) / np.max(np.abs(xd)) < 1e-7
+
+
+@pytest.mark.parametrize("vcycle", ["fp64", "fp32"])
+def test_block_coarse_levels(rf, ora, vcycle):
+    """Three smoothed levels of a C3D8 block: aggregates of 6 rigid body modes
+    make 6 x 6 block matrices on the coarse levels; the solve still meets the
+    FP64 target and the direct solution."""
+    p = meshes.hex_block(20, nrhs=1)
+    print(f"case block coarse levels hex20 n={p.n} {vcycle}", flush=True)
+    s, _ = setup(rf, p, krylov="fcg" if vcycle == "fp32" else "pcg", vcycle=vcycle)
+    st = s.stats()
+    assert st["levels"] >= 3 and st["blocks"][:3] == [3, 6, 6]
+    b = p.rhs[:, 0].copy()
+    x, it, rel, conv = s.solve(b, tol=1e-10)
+    assert conv and rel <= 1e-10
+    a = p.to_scipy()
+    assert np.linalg.norm(a @ x - b) / np.linalg.norm(b) < 1e-9
+    h, _, _ = ora.Handle.factor(p)
+    xd = h.solve(b)
+    assert np.max(np.abs(x - xd)) / np.max(np.abs(xd)) < 1e-8
+    print(f"iterations {it} sizes {st['sizes']} blocks {st['blocks']}", flush=True)
 
 
 def test_jacobi_pcg_matches_oracle_iterations(rf, ora):
