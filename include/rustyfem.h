@@ -194,6 +194,14 @@ int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol,
  * of a 3D hierarchy; 1 = scalar CSR). */
 void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
 
+/* Coefficients of the V-cycle's Chebyshev smoother (additive, no counterpart
+ * in the reference, whose smoother is Gauss-Seidel): degree steps on
+ * [hi / ratio, hi] of D^-1 A, d_0 = c0 D^-1 b, then d_k = c1[k-1] d_{k-1} +
+ * c2[k-1] D^-1 (b - A x_k) and x += d_k; the V-cycle uses hi = 1.1 x the
+ * power-method radius.  c1, c2 hold degree - 1 entries.  0, or -1 for invalid
+ * arguments. */
+int32_t rustyfem_chebyshev(double hi, double ratio, int32_t degree, double *c0, double *c1, double *c2);
+
 /* z = M r: one application of the handle's preconditioner (Precond::apply,
  * src/solver/iterative/csr.rs:193-196; for IC(0) csr.rs:296-316, for Jacobi
  * csr.rs:209-215).  r and z are host pointers, or device pointers when

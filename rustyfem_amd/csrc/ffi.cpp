@@ -20,6 +20,7 @@
 #include "indef.hpp"
 #include "mf.hpp"
 #include "mpnn_train.hpp"
+#include "chebyshev.hpp"
 #include "pcg.hpp"
 #include "symbolic.hpp"
 
@@ -694,6 +695,17 @@ int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol,
 }
 
 void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) { h->solver->stats(out, len); }
+
+int32_t rustyfem_chebyshev(double hi, double ratio, int32_t degree, double* c0, double* c1, double* c2) {
+  if (!(hi > 0.0) || !(ratio > 1.0) || degree < 1) return -1;
+  std::vector<std::pair<double, double>> steps;
+  rf::chebyshev<double>(hi, ratio, degree, *c0, steps);
+  for (size_t k = 0; k < steps.size(); ++k) {
+    c1[k] = steps[k].first;
+    c2[k] = steps[k].second;
+  }
+  return 0;
+}
 
 int32_t rustyfem_pcg_precondition(PcgHandle* h, const double* r, double* z, int32_t on_device) {
   try {

@@ -27,6 +27,7 @@ constexpr int TRS = 64;       // rows per TRSM tile
 constexpr int SR = 1024;      // rows per solve tile (4 per thread)
 constexpr int BC = 256;       // columns per backward tile (one per thread)
 constexpr int GT_M = 128;     // GEMM tile rows
+constexpr int ZC = 64;        // columns per arena-clearing tile
 constexpr int EA_T = 64;      // extend-add tile edge
 constexpr int SBW = 512;      // super-block width of the triangular solves
 constexpr int SRB = 64;       // rows per tile of the forward super-block update
@@ -40,6 +41,7 @@ struct FrontDesc {
   int32_t cmap_off;  // offset of this front's child->parent row map (length u), -1 for a root
 };
 
+struct ZeroTile { int32_t front, c0; };
 struct ScatterTile { int64_t p0, p1; };
 struct EaTile { int32_t child, x0, y0, pad; };
 // C[r0:r0+128, c0:min(c0+TN, cend)) -= L[r0:, k0:k1] D[k0:k1] L[c0:, k0:k1]^T
@@ -126,6 +128,7 @@ struct Plan {
   std::vector<int32_t> level_fronts;  // fronts grouped by level (solve kernels)
   std::vector<int32_t> cmap;          // child row -> parent local index
   std::vector<int64_t> amap;          // Lower entry -> arena offset
+  std::vector<ZeroTile> zero;          // lower triangles of all fronts, ZC columns per tile
   std::vector<ScatterTile> scatter;
   std::vector<EaTile> ea;
   std::vector<GemmTile> gemm;

@@ -45,19 +45,29 @@ namespace {
 
 // ---------------------------------------------------------------- assembly
 
-// Clears the whole arena (every front's m x m block) once per factorization:
-// one grid-stride stream of 16-byte stores, HBM-bound.  The arena is not
-// reused across levels, so nothing written later is cleared by it.
+// Clears the lower triangle of every front (column c: rows c..m-1, one
+// contiguous segment of the column-major front) once per factorization, one
+// wave per column, 16-byte stores between the unaligned ends.  The arena is
+// not reused across levels, so nothing written later is cleared by it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_clear(T* __restrict__ arena, int64_t n) {
-  const int64_t nv = n * static_cast<int64_t>(sizeof(T)) / 16;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
-  float4* a4 = reinterpret_cast<float4*>(arena);
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < nv; i += stride)
-    a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int64_t i = nv * 16 / static_cast<int64_t>(sizeof(T)) + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-       i < n; i += stride)
-    arena[i] = T(0);
+__global__ __launch_bounds__(256) void k_zero(const FrontDesc* __restrict__ fd, const ZeroTile* __restrict__ tiles,
+                                              T* __restrict__ arena) {
+  constexpr int V = 16 / static_cast<int>(sizeof(T));
+  const ZeroTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cend = min(t.c0 + ZC, d.m);
+  for (int c = t.c0 + wave; c < cend; c += 4) {
+    const int64_t base = d.off + static_cast<int64_t>(c) * d.m;
+    const int64_t s = base + c, e = base + d.m;
+    const int64_t a = min(e, (s + V - 1) / V * V);  // first 16-byte aligned element
+    const int64_t b = max(a, e / V * V);             // end of the aligned body
+    if (lane < a - s) arena[s + lane] = T(0);
+    if (lane < e - b) arena[b + lane] = T(0);
+    float4* p = reinterpret_cast<float4*>(arena + a);
+    const int64_t nv = (b - a) / V;
+    for (int64_t i = lane; i < nv; i += 64) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 template <typename T>
@@ -782,7 +792,7 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (valid) {
-#pragma unroll 16
+#pragma unroll 32
       for (int c = cb; c < ce; ++c) {
         const double l = static_cast<double>(F[static_cast<int64_t>(c) * d.m]);
 #pragma unroll
@@ -1091,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ 
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* xr = X + i;
-#pragma unroll 8
+#pragma unroll 32
       for (int q = cb; q < ce; ++q) {
         const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
@@ -1131,6 +1141,8 @@ __global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j0) * BC + cl;
+      // the loads of eight row chunks are issued together
+#pragma unroll 8
       for (int rc = rb; rc < re; ++rc)
 #pragma unroll
         for (int j = 0; j < R; ++j)
@@ -1180,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ 
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* xr = Xt + i;
-#pragma unroll 8
+#pragma unroll 32
       for (int q = cb; q < ce; ++q) {
         const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
@@ -1335,6 +1347,7 @@ class Impl final : public GpuFactor {
     d_fd_ = dupload(P_.fd, bytes_, s_);
     d_cmap_ = dupload(P_.cmap, bytes_, s_);
     d_amap_ = dupload(P_.amap, bytes_, s_);
+    d_zero_ = dupload(P_.zero, bytes_, s_);
     d_scatter_ = dupload(P_.scatter, bytes_, s_);
     d_ea_ = dupload(P_.ea, bytes_, s_);
     d_gemm_ = dupload(P_.gemm, bytes_, s_);
@@ -1417,7 +1430,7 @@ class Impl final : public GpuFactor {
   ~Impl() override {
     (void)hipStreamSynchronize(s_);
     drop_graphs();
-    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
+    void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
@@ -1438,11 +1451,7 @@ class Impl final : public GpuFactor {
     if (autotune_ && !tuned_) tune_gemm();
     nev_ = 0;
     tm_.gemm_flops = 0;
-    {
-      const int64_t nv = P_.arena_elems * static_cast<int64_t>(sizeof(T)) / 16;
-      launch(k_clear<T>, std::max<int64_t>(1, std::min<int64_t>(8192, (nv + 255) / 256)), 256, d_arena_,
-             P_.arena_elems);
-    }
+    launch(k_zero<T>, static_cast<int64_t>(P_.zero.size()), 256, d_fd_, d_zero_, d_arena_);
     for (const LevelPlan& lp : P_.levels) {
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
@@ -1905,6 +1914,7 @@ class Impl final : public GpuFactor {
   FrontDesc* d_fd_ = nullptr;
   int32_t* d_cmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
+  ZeroTile* d_zero_ = nullptr;
   ScatterTile* d_scatter_ = nullptr;
   EaTile* d_ea_ = nullptr;
   GemmTile* d_gemm_ = nullptr;

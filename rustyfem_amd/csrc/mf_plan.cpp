@@ -51,6 +51,8 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     P.flops += k * k * k / 3.0 + uu * k * k + uu * uu * k;
   }
   P.arena_elems = off;
+  for (int64_t f = 0; f < P.nf; ++f)
+    for (int32_t c0 = 0; c0 < P.fd[f].m; c0 += ZC) P.zero.push_back({static_cast<int32_t>(f), c0});
   P.tvec_elems = toff;
   // child row -> parent local index
   P.cmap.resize(ncmap);

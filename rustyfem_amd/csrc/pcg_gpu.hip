@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "amg.hpp"
+#include "chebyshev.hpp"
 #include "ic0.hpp"
 #include "mpnn.hpp"
 #include "pcg.hpp"
@@ -76,9 +77,24 @@ __device__ __forceinline__ double sum_partials(const double* part, double* sh) {
 // ---------------------------------------------------------------------------
 // block-row sparse products: MODE 0 y = A x, 1 y = b - A x, 2 y += A x
 
-// Blocks are BR x BC, row-major: y[BR g + i] += sum_j v[i][j] x[BC c + j].
-template <typename T, int BR, int BC, int G, int MODE>
-__global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, const int64_t* __restrict__ ptr,
+// Entries per 16-byte granule of the interleaved layout: 16 bytes when the
+// block's entry count allows, else 8, else single entries.
+constexpr int granule(int bb, int tsize) {
+  return bb % (16 / tsize) == 0 ? 16 / tsize : (8 / tsize >= 1 && bb % (8 / tsize) == 0 ? 8 / tsize : 1);
+}
+
+template <typename T, int N>
+struct Vec {
+  typedef T type __attribute__((ext_vector_type(N)));
+};
+
+// Blocks are BR x BC: y[BR g + i] += sum_j v[i][j] x[BC c + j].  Block q's
+// entry w = BC i + j is val[q BR BC + w] (GR = 0: block-major) or
+// val[((w / GR) nnzb + q) GR + w % GR] (granules of GR entries interleaved
+// across blocks: the G lanes of a row, which take consecutive blocks, read
+// consecutive granules, one vector load each).
+template <typename T, int BR, int BC, int G, int MODE, int GR>
+__global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, int64_t nnzb, const int64_t* __restrict__ ptr,
                                              const int32_t* __restrict__ idx, const T* __restrict__ val,
                                              const T* __restrict__ x, T* y, const T* b, const int32_t* done) {
   if (done && *done) return;
@@ -90,14 +106,23 @@ __global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, const int64_t* __restr
   for (int i = 0; i < BR; ++i) s[i] = T(0);
   for (int64_t q = ptr[g] + lane; q < ptr[g + 1]; q += G) {
     const int64_t c = idx[q];
-    const T* v = val + q * (BR * BC);
     T xv[BC];
 #pragma unroll
     for (int j = 0; j < BC; ++j) xv[j] = x[c * BC + j];
+    if constexpr (GR == 0) {
 #pragma unroll
-    for (int i = 0; i < BR; ++i)
+      for (int i = 0; i < BR; ++i)
 #pragma unroll
-      for (int j = 0; j < BC; ++j) s[i] += v[i * BC + j] * xv[j];
+        for (int j = 0; j < BC; ++j) s[i] += val[q * (BR * BC) + i * BC + j] * xv[j];
+    } else {
+      using VT = typename Vec<T, GR>::type;
+#pragma unroll
+      for (int gi = 0; gi < BR * BC / GR; ++gi) {
+        const VT v = *reinterpret_cast<const VT*>(val + (static_cast<int64_t>(gi) * nnzb + q) * GR);
+#pragma unroll
+        for (int e = 0; e < GR; ++e) s[(gi * GR + e) / BC] += v[e] * xv[(gi * GR + e) % BC];
+      }
+    }
   }
 #pragma unroll
   for (int i = 0; i < BR; ++i)
@@ -441,6 +466,7 @@ struct DevMat {
   int bs = 1;                // rows per block
   int bsc = 1;               // columns per block
   int group = 8;             // lanes per block row
+  bool soa = false;          // granule-interleaved block values (see k_spmv)
   int64_t nnzb = 0;
   DBuf ptr, idx, v64, v32;
 
@@ -468,11 +494,12 @@ int pick_group(double per_row) {
 
 // scalar CSR -> bs x bsc blocks (every entry placed, missing ones zero); the
 // row and column counts must be multiples of the block sizes
-DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32) {
+DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32, bool soa = false) {
   if (a.nrows % bs || a.ncols % bsc) throw std::runtime_error("internal: block sizes do not divide the matrix");
   DevMat M;
   M.bs = bs;
   M.bsc = bsc;
+  M.soa = soa && bs * bsc > 1;
   M.nbr = a.nrows / bs;
   M.nbc = a.ncols / bsc;
   std::vector<int64_t> bptr(M.nbr + 1, 0);
@@ -506,9 +533,19 @@ DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32) {
   M.group = pick_group(M.nbr ? static_cast<double>(M.nnzb) / M.nbr : 1.0);
   M.ptr = upload(bptr);
   M.idx = upload(bidx);
-  if (keep64) M.v64 = upload(bval);
+  const int bb = bs * bsc;
+  // interleaved layout with the granule of each value type (see k_spmv)
+  auto interleave = [&](int gr) {
+    std::vector<double> t(bval.size());
+    for (int64_t k = 0; k < M.nnzb; ++k)
+      for (int w = 0; w < bb; ++w)
+        t[(static_cast<size_t>(w / gr) * M.nnzb + k) * gr + w % gr] = bval[static_cast<size_t>(k) * bb + w];
+    return t;
+  };
+  if (keep64) M.v64 = upload(M.soa ? interleave(granule(bb, 8)) : bval);
   if (keep32) {
-    std::vector<float> f(bval.begin(), bval.end());
+    const std::vector<double> t = M.soa ? interleave(granule(bb, 4)) : bval;
+    std::vector<float> f(t.begin(), t.end());
     M.v32 = upload(f);
   }
   return M;
@@ -526,9 +563,17 @@ void launch_spmv_g(const DevMat& M, int mode, const T* x, T* y, const T* b, cons
   const unsigned grid = grid_for(M.nbr * G);
   const auto* P = M.ptr.as<int64_t>();
   const auto* I = M.idx.as<int32_t>();
-  if (mode == 0) k_spmv<T, BR, BC, G, 0><<<grid, RB, 0, s>>>(M.nbr, P, I, vals<T>(M), x, y, b, done);
-  else if (mode == 1) k_spmv<T, BR, BC, G, 1><<<grid, RB, 0, s>>>(M.nbr, P, I, vals<T>(M), x, y, b, done);
-  else k_spmv<T, BR, BC, G, 2><<<grid, RB, 0, s>>>(M.nbr, P, I, vals<T>(M), x, y, b, done);
+  const T* V = vals<T>(M);
+  if (M.soa && BR * BC > 1) {
+    constexpr int S = granule(BR * BC, static_cast<int>(sizeof(T)));
+    if (mode == 0) k_spmv<T, BR, BC, G, 0, S><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
+    else if (mode == 1) k_spmv<T, BR, BC, G, 1, S><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
+    else k_spmv<T, BR, BC, G, 2, S><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
+    return;
+  }
+  if (mode == 0) k_spmv<T, BR, BC, G, 0, 0><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
+  else if (mode == 1) k_spmv<T, BR, BC, G, 1, 0><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
+  else k_spmv<T, BR, BC, G, 2, 0><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, V, x, y, b, done);
 }
 
 template <typename T, int BR, int BC>
@@ -704,22 +749,6 @@ double PcgImpl::power_radius(const DevMat& M, const std::vector<double>& diag) {
   return rho * 1.05;
 }
 
-// Chebyshev coefficients for D^-1 A on [hi / ratio, hi] (three-term recurrence)
-template <typename T>
-void chebyshev(double hi, double ratio, int degree, T& c0, std::vector<std::pair<T, T>>& steps) {
-  const double lo = hi / ratio;
-  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
-  const double sigma = theta / delta;
-  double rho_old = 1.0 / sigma;
-  c0 = static_cast<T>(1.0 / theta);
-  steps.clear();
-  for (int k = 1; k < degree; ++k) {
-    const double rho_new = 1.0 / (2.0 * sigma - rho_old);
-    steps.push_back({static_cast<T>(rho_new * rho_old), static_cast<T>(2.0 * rho_new / delta)});
-    rho_old = rho_new;
-  }
-}
-
 template <typename T>
 void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb) {
   const bool f32 = std::is_same<T, float>::value;
@@ -732,8 +761,8 @@ void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& c
     // transfer operators in blocks of the two levels' DOFs per node/aggregate
     const int bf = blk_[l], bcs = blk_[l + 1];
     const bool ok = spmv_shape_ok(bf, bcs) && spmv_shape_ok(bcs, bf);
-    V.P = make_mat(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32);
-    V.R = make_mat(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32);
+    V.P = make_mat(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32, true);
+    V.R = make_mat(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32, true);
     std::vector<T> di(V.n);
     for (int64_t i = 0; i < V.n; ++i) di[i] = static_cast<T>(1.0 / L.diag[i]);
     V.dinv = upload(di);
@@ -802,7 +831,7 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
     for (size_t l = 1; l < H.levels.size(); ++l) blk_.push_back(uniform_block(H.levels[l].bptr));
     blk_.push_back(uniform_block(H.coarse_bptr));
     for (size_t l = 1; l < H.levels.size(); ++l)
-      coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32));
+      coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true));
     if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
     else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
   }

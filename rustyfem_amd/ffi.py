@@ -25,7 +25,7 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_factor_batch", "rustyfem_refactor_batch", "rustyfem_solve_batch", "rustyfem_solve_batch_device",
            "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch",
            "rustyfem_pcg_set_learned_smoother", "rustyfem_mpnn_nparams", "rustyfem_mpnn_create", "rustyfem_mpnn_loss",
-           "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition"]
+           "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition", "rustyfem_chebyshev"]
 
 _lib = None
 
@@ -91,6 +91,8 @@ def lib():
         L.rustyfem_pcg_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int64, C.c_int32, f64p, C.POINTER(C.c_int32)]
         L.rustyfem_pcg_stats.restype = None
         L.rustyfem_pcg_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_chebyshev.restype = C.c_int32
+        L.rustyfem_chebyshev.argtypes = [C.c_double, C.c_double, C.c_int32, vp, vp, vp]
         L.rustyfem_pcg_precondition.restype = C.c_int32
         L.rustyfem_pcg_precondition.argtypes = [vp, vp, vp, C.c_int32]
         L.rustyfem_pcg_free.restype = None
@@ -343,6 +345,11 @@ class Pcg:
             raise RuntimeError(last_error())
         return z
 
+    def precondition_device(self, d_r, d_z):
+        """z = M r on device pointers."""
+        if lib().rustyfem_pcg_precondition(self._h, d_r, d_z, 1) != 0:
+            raise RuntimeError(last_error())
+
     def set_learned_smoother(self, params, sweeps=1):
         """Finest-level smoother x += M (b - A x) with M from the learned network."""
         params = np.ascontiguousarray(params, dtype=np.float64)
@@ -353,6 +360,17 @@ class Pcg:
         if getattr(self, "_h", None):
             lib().rustyfem_pcg_free(self._h)
             self._h = None
+
+
+def chebyshev(hi, ratio, degree):
+    """(c0, c1, c2) of the V-cycle's Chebyshev smoother (rustyfem_chebyshev)."""
+    c0 = np.zeros(1)
+    c1 = np.zeros(max(1, degree - 1))
+    c2 = np.zeros(max(1, degree - 1))
+    if lib().rustyfem_chebyshev(float(hi), float(ratio), int(degree), c0.ctypes.data, c1.ctypes.data,
+                                c2.ctypes.data) != 0:
+        raise ValueError("invalid Chebyshev interval or degree")
+    return float(c0[0]), c1[:degree - 1], c2[:degree - 1]
 
 
 def mpnn_nparams():
