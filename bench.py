@@ -504,6 +504,7 @@ def main():
                    "levels": st["levels"]},
         "mixed": {"time_s": mixed["time_s"], "factor_ms": st["factor_ms"], "solve_ms": st["solve_ms"],
                   "refine_steps": st["refine_steps"], "rel_residual": st["rel_residual"],
+                  "refine_history": st["refine_history"],
                   "true_rel_residual": mixed["true_rel_residual"], "factor_tflops": factor_tflops,
                   "factor_frac_fp32_peak": factor_tflops / PEAK["fp32_mfma_tflops"]},
         "roofline": roof,

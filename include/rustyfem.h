@@ -101,7 +101,9 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
  * environment variable RUSTYFEM_KERNEL_TIMING=1 was set at factorization;
  * [13] time (ms) of the GEMM autotuning done before the first factorization
  * (RUSTYFEM_AUTOTUNE=0 turns it off), [14] the GEMM launches it timed and
- * [15] how many of them it moved to the AGPR-accumulator kernel. */
+ * [15] how many of them it moved to the AGPR-accumulator kernel,
+ * [16..19] the worst relative residual after 0, 1, 2 and 3 refinement
+ * corrections of the last solve (-1 where not reached). */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */

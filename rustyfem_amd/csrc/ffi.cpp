@@ -504,7 +504,7 @@ void rustyfem_batch_stats(const BatchHandle* h, double* out, size_t len) {
   v[13] = t.tune_ms;
   v[14] = t.tuned_launches;
   v[15] = t.agpr_launches;
-  for (size_t i = 0; i < len && i < 16; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < 20; ++i) out[i] = v[i];
 }
 
 void rustyfem_free_batch(BatchHandle* h) { delete h; }
@@ -608,7 +608,7 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
 }
 
 void rustyfem_stats(const Handle* h, double* out, size_t len) {
-  double v[16] = {0};
+  double v[20] = {0};
   if (h->gpu) {
     const rf::Timings& t = h->gpu->timings();
     v[0] = t.factor_ms;
@@ -624,11 +624,12 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
     v[13] = t.tune_ms;
     v[14] = t.tuned_launches;
     v[15] = t.agpr_launches;
+    for (int i = 0; i < 4; ++i) v[16 + i] = t.refine_history[i];
   }
   v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
   v[8] = static_cast<double>(h->nfronts);
   v[9] = static_cast<double>(h->nlevels);
-  for (size_t i = 0; i < len && i < 16; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < 20; ++i) out[i] = v[i];
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }

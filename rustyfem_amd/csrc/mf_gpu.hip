@@ -792,7 +792,7 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (valid) {
-#pragma unroll 32
+#pragma unroll 16
       for (int c = cb; c < ce; ++c) {
         const double l = static_cast<double>(F[static_cast<int64_t>(c) * d.m]);
 #pragma unroll
@@ -1101,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ 
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* xr = X + i;
-#pragma unroll 32
+#pragma unroll 8
       for (int q = cb; q < ce; ++q) {
         const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ 
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* xr = Xt + i;
-#pragma unroll 32
+#pragma unroll 8
       for (int q = cb; q < ce; ++q) {
         const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
@@ -1374,6 +1374,8 @@ class Impl final : public GpuFactor {
       legacy_panel_ = q && *q && *q != '0';
       const char* c = std::getenv("RUSTYFEM_SOLVE_CHAIN");
       chain_solve_ = c && *c && *c != '0';
+      const char* rb = std::getenv("RUSTYFEM_SOLVE_RHS_BLOCK");
+      if (rb && *rb) rhs_block_ = std::atoi(rb);
       // RUSTYFEM_AUTOTUNE: unset or 1 = time both GEMM kernels per launch,
       // 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere
       const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
@@ -1697,6 +1699,7 @@ class Impl final : public GpuFactor {
     norms(d_b_, bn.data());
     int steps = 0;
     double worst = 0.0, prev = HUGE_VAL;
+    for (double& h : tm_.refine_history) h = -1.0;
     for (;;) {
       launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, d_x_, d_b_, d_r_, n_, nrhs);
       norms(d_r_, rn.data());
@@ -1707,6 +1710,7 @@ class Impl final : public GpuFactor {
         worst = std::max(worst, rel);
         if (!(rel <= tol)) done = false;
       }
+      if (steps < 4) tm_.refine_history[steps] = worst;
       // stop at the target, at the step limit, or when a correction no longer
       // halves the worst residual (the FP64 rounding floor of the residual)
       if (done || steps >= max_steps || !(worst < 0.5 * prev)) break;
@@ -1734,10 +1738,13 @@ class Impl final : public GpuFactor {
     HIPCHK(hipStreamSynchronize(s_));
   }
 
+  // Several right-hand sides go through the kernels in passes of 4 by
+  // default: a pass of 16 needs about 100 KB of LDS per workgroup (one per
+  // CU), which the many small tiles of the lower levels cannot afford.
   void solve_eager(double* dx, int nrhs) {
     if (nrhs == 1)
       solve_levels<1>(dx, nrhs);
-    else if (nrhs <= 4)
+    else if (nrhs <= 4 || rhs_block_ != 16)
       solve_levels<4>(dx, nrhs);
     else
       solve_levels<16>(dx, nrhs);
@@ -1937,6 +1944,7 @@ class Impl final : public GpuFactor {
   bool autotune_ = true, tuned_ = false;
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): 1 = k_gemm_agpr
   bool chain_solve_ = false;
+  int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (4 or 16)
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;
   double* d_xinv_ = nullptr;

@@ -273,12 +273,14 @@ class Handle:
         return o
 
     def stats(self):
-        o = np.zeros(16)
-        lib().rustyfem_stats(self._h, o.ctypes.data, 16)
+        o = np.zeros(20)
+        lib().rustyfem_stats(self._h, o.ctypes.data, 20)
         keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
                 "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
                 "tuned_launches", "agpr_launches"]
-        return dict(zip(keys, o.tolist()))
+        d = dict(zip(keys, o.tolist()))
+        d["refine_history"] = [float(v) for v in o[16:20] if v >= 0]
+        return d
 
 
 KIND = {"amg": 0, "ic0": 1, "jacobi": 2}
