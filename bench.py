@@ -190,7 +190,8 @@ def run_learned(args):
     from x0 = 0 with the preconditioner set up and b in HBM."""
     import numpy as np
     from rustyfem_amd import ffi, hip, meshes
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rustyfem_amd", "data", "smoother_c3d4.npz")
+    path = args.smoother or os.path.join(os.path.dirname(os.path.abspath(__file__)), "rustyfem_amd", "data",
+                                         "smoother_c3d4.npz")
     with np.load(path, allow_pickle=False) as z:
         params = z["params"].copy()
         meta = json.loads(str(z["meta"]))
@@ -433,6 +434,7 @@ def main():
                          "batch: 512 plates factored and solved together; learned: the C3D4 block with the "
                          "learned-smoother FCG")
     ap.add_argument("--tet-size", type=int, default=96, help="cubes per edge of the C3D4 block (learned workload)")
+    ap.add_argument("--smoother", default=None, help="learned-smoother parameters (.npz; default rustyfem_amd/data)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied device-to-device twice before the run (a kernel of known traffic for "
