@@ -372,14 +372,18 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
 // block's triangular solve is a product with its FP64 inverse V = L_BB^-1.
 
 constexpr int RB = 16;  // right-hand sides per pass in the solve kernels
+// Entries per workgroup of the per-front vector kernels (k_fwd_init,
+// k_fwd_child, k_*_finish, k_bwd_init): their grids are (front, right-hand
+// side, chunk of VCH entries), so the large fronts of the upper levels and
+// many right-hand sides spread over the whole GPU.
+constexpr int VCH = 2048;
 
 __global__ __launch_bounds__(256) void k_fwd_init(const FrontDesc* __restrict__ fd, const int32_t* __restrict__ fronts,
                                                   const double* __restrict__ x, double* __restrict__ t, int64_t n,
                                                   int64_t tv, int nrhs) {
   const FrontDesc d = fd[fronts[blockIdx.x]];
-  for (int j = 0; j < nrhs; ++j)
-    for (int i = threadIdx.x; i < d.m; i += 256)
-      t[j * tv + d.toff + i] = i < d.k ? x[j * n + d.start + i] : 0.0;
+  const int j = blockIdx.y, i0 = blockIdx.z * VCH, i1 = min(d.m, i0 + VCH);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) t[j * tv + d.toff + i] = i < d.k ? x[j * n + d.start + i] : 0.0;
 }
 
 // t_parent[cmap[x]] += t_child[k_child + x] (forward: the child's update vector)
@@ -389,8 +393,8 @@ __global__ __launch_bounds__(256) void k_fwd_child(const FrontDesc* __restrict__
   const FrontDesc c = fd[children[blockIdx.x]];
   const FrontDesc p = fd[c.parent];
   const int32_t* cm = cmap + c.cmap_off;
-  for (int j = 0; j < nrhs; ++j)
-    for (int x = threadIdx.x; x < c.u; x += 256) t[j * tv + p.toff + cm[x]] += t[j * tv + c.toff + c.k + x];
+  const int j = blockIdx.y, x0 = blockIdx.z * VCH, x1 = min(c.u, x0 + VCH);
+  for (int x = x0 + threadIdx.x; x < x1; x += 256) t[j * tv + p.toff + cm[x]] += t[j * tv + c.toff + c.k + x];
 }
 
 // Vs <- V of one pivot block (global -> LDS)
@@ -493,9 +497,8 @@ __global__ __launch_bounds__(256) void k_fwd_finish(const FrontDesc* __restrict_
                                                     const double* __restrict__ t, const double* __restrict__ dpiv,
                                                     double* __restrict__ x, int64_t n, int64_t tv, int nrhs) {
   const FrontDesc d = fd[fronts[blockIdx.x]];
-  for (int j = 0; j < nrhs; ++j)
-    for (int i = threadIdx.x; i < d.k; i += 256)
-      x[j * n + d.start + i] = t[j * tv + d.toff + i] / dpiv[d.start + i];
+  const int j = blockIdx.y, i0 = blockIdx.z * VCH, i1 = min(d.k, i0 + VCH);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) x[j * n + d.start + i] = t[j * tv + d.toff + i] / dpiv[d.start + i];
 }
 
 // backward: t[0:k] = x[pivots], t[k + r] = t_parent[cmap[r]] (final values of ancestors)
@@ -503,17 +506,17 @@ __global__ __launch_bounds__(256) void k_bwd_init(const FrontDesc* __restrict__ 
                                                   const int32_t* __restrict__ cmap, const double* __restrict__ x,
                                                   double* __restrict__ t, int64_t n, int64_t tv, int nrhs) {
   const FrontDesc d = fd[fronts[blockIdx.x]];
-  for (int j = 0; j < nrhs; ++j)
-    for (int i = threadIdx.x; i < d.m; i += 256) {
-      double v;
-      if (i < d.k) {
-        v = x[j * n + d.start + i];
-      } else {
-        const FrontDesc p = fd[d.parent];
-        v = t[j * tv + p.toff + cmap[d.cmap_off + i - d.k]];
-      }
-      t[j * tv + d.toff + i] = v;
+  const int j = blockIdx.y, i0 = blockIdx.z * VCH, i1 = min(d.m, i0 + VCH);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    double v;
+    if (i < d.k) {
+      v = x[j * n + d.start + i];
+    } else {
+      const FrontDesc p = fd[d.parent];
+      v = t[j * tv + p.toff + cmap[d.cmap_off + i - d.k]];
     }
+    t[j * tv + d.toff + i] = v;
+  }
 }
 
 // partial[tile][j][c] = sum over rows r of the tile of L21[r, c] t[r][j] for
@@ -686,8 +689,8 @@ __global__ __launch_bounds__(256) void k_bwd_finish(const FrontDesc* __restrict_
                                                     const double* __restrict__ t, double* __restrict__ x, int64_t n,
                                                     int64_t tv, int nrhs) {
   const FrontDesc d = fd[fronts[blockIdx.x]];
-  for (int j = 0; j < nrhs; ++j)
-    for (int i = threadIdx.x; i < d.k; i += 256) x[j * n + d.start + i] = t[j * tv + d.toff + i];
+  const int j = blockIdx.y, i0 = blockIdx.z * VCH, i1 = min(d.k, i0 + VCH);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) x[j * n + d.start + i] = t[j * tv + d.toff + i];
 }
 
 // ------------------------------------------------ super-block triangular solves
@@ -780,7 +783,7 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
   double* tf = t + d.toff;
   const double* yf = ysrc + d.toff;
   const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + r;
-  for (int j0 = 0; j0 < nrhs; j0 += R) {
+  for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
     for (int idx = tid; idx < W * R; idx += 256) {
@@ -852,7 +855,7 @@ __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__
   int rowc[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) rowc[s] = min(tl.r0 + rl + 16 * s, rend - 1);
-  for (int j0 = 0; j0 < nrhs; j0 += R) {
+  for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
     double tr[4][R];
 #pragma unroll
@@ -1088,7 +1091,7 @@ __global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ 
   const double* X = xinv + sboff[tk.sb];
   const double* tf = t + d.toff + tk.J0;
   double* yf = ys + d.toff + tk.J0;
-  for (int j0 = 0; j0 < nrhs; j0 += R) {
+  for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
     for (int idx = tid; idx < jend * R; idx += 256) {
@@ -1134,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__
   const int qw = (tk.nrc + 3) / 4, rb = w * qw, re = min(tk.nrc, rb + qw);
   const double* tf = t + d.toff + tk.J0;
   double* zf = ys + d.toff + tk.J0;
-  for (int j0 = 0; j0 < nrhs; j0 += R) {
+  for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
     double acc[R];
 #pragma unroll
@@ -1179,7 +1182,7 @@ __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ 
   const double* Xt = xinvt + sboff[tk.sb];
   const double* zf = ys + d.toff + tk.J0;
   double* tf = t + d.toff + tk.J0;
-  for (int j0 = 0; j0 < nrhs; j0 += R) {
+  for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
     for (int idx = tid; idx < span * R; idx += 256) {
@@ -1213,9 +1216,8 @@ __global__ __launch_bounds__(256) void k_fwd_finish_ys(const FrontDesc* __restri
                                                        const double* __restrict__ ys, const double* __restrict__ dpiv,
                                                        double* __restrict__ x, int64_t n, int64_t tv, int nrhs) {
   const FrontDesc d = fd[fronts[blockIdx.x]];
-  for (int j = 0; j < nrhs; ++j)
-    for (int i = threadIdx.x; i < d.k; i += 256)
-      x[j * n + d.start + i] = ys[j * tv + d.toff + i] / dpiv[d.start + i];
+  const int j = blockIdx.y, i0 = blockIdx.z * VCH, i1 = min(d.k, i0 + VCH);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) x[j * n + d.start + i] = ys[j * tv + d.toff + i] / dpiv[d.start + i];
 }
 
 // ------------------------------------------------------------- refinement
@@ -1385,6 +1387,22 @@ class Impl final : public GpuFactor {
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
+    // longest front vectors per level (grid extents of the per-front vector kernels)
+    lvl_maxm_.assign(P_.levels.size(), 0);
+    lvl_maxk_.assign(P_.levels.size(), 0);
+    for (size_t L = 0; L < P_.levels.size(); ++L)
+      for (int32_t f : P_.levels[L].fronts) {
+        lvl_maxm_[L] = std::max<int64_t>(lvl_maxm_[L], P_.fd[f].m);
+        lvl_maxk_[L] = std::max<int64_t>(lvl_maxk_[L], P_.fd[f].k);
+      }
+    for (int sl = 0; sl < 2; ++sl) {
+      slot_maxu_[sl].assign(P_.levels.size(), 0);
+      for (size_t L = 0; L < P_.levels.size() && L < P_.child_slot_range[sl].size(); ++L) {
+        const Range& cr = P_.child_slot_range[sl][L];
+        for (int64_t q = cr.begin; q < cr.begin + cr.count; ++q)
+          slot_maxu_[sl][L] = std::max<int64_t>(slot_maxu_[sl][L], P_.fd[P_.child_slot_fronts[sl][q]].u);
+      }
+    }
     d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems), bytes_);
     d_dpiv_ = dalloc<double>(n_, bytes_);
     d_adiag_ = dalloc<double>(n_, bytes_);
@@ -1569,6 +1587,16 @@ class Impl final : public GpuFactor {
   void launch(K kernel, int64_t blocks, int threads, A... args) {
     if (blocks <= 0) return;
     hipLaunchKernelGGL(kernel, dim3(grid(blocks)), dim3(threads), 0, s_, args...);
+    HIPCHK(hipGetLastError());
+  }
+
+  // a per-front vector kernel over `fronts` fronts, nrhs right-hand sides and
+  // entries in chunks of VCH up to maxlen (the longest front vector of the launch)
+  template <typename K, typename... A>
+  void launch_vec(K kernel, int64_t fronts, int nrhs, int64_t maxlen, A... args) {
+    if (fronts <= 0 || nrhs <= 0 || maxlen <= 0) return;
+    const unsigned nch = static_cast<unsigned>((maxlen + VCH - 1) / VCH);
+    hipLaunchKernelGGL(kernel, dim3(grid(fronts), static_cast<unsigned>(nrhs), nch), dim3(256), 0, s_, args...);
     HIPCHK(hipGetLastError());
   }
 
@@ -1795,10 +1823,11 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_fwd_init, lp.fronts_r.count, 256, d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+                   nrhs);
       }
       if (!lp.solve.empty())
         launch(k_fwd_first<R>, lp.solve[0].diag.count, 256, d_fd_, d_diag_, lp.solve[0].diag.begin, d_vbuf_, d_t_,
@@ -1806,19 +1835,19 @@ class Impl final : public GpuFactor {
       for (const SolveStep& sp : lp.solve)
         launch(k_fwd_step<T, R>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_vbuf_, d_t_, tv,
                nrhs);
-      launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const BwdPlan& bp = lp.bwd;
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
       launch(k_l21_partial<T, R>, bp.l21.count, 256, d_fd_, d_l21_ + bp.l21.begin, d_arena_, d_t_, tv, nrhs, d_part_);
       launch(k_l21_combine, bp.comb.count, 256, d_fd_, d_l21c_ + bp.comb.begin, d_part_, d_t_, tv, nrhs);
       launch(k_bwd_last<R>, bp.last.count, 256, d_fd_, d_diag_, d_last_ + bp.last.begin, d_vbuf_, d_t_, tv, nrhs);
       for (const Range& st : bp.steps)
         launch(k_bwd_step<T, R>, st.count, 256, d_fd_, d_btile_ + st.begin, d_arena_, d_vbuf_, d_t_, tv, nrhs);
-      launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
     }
   }
 
@@ -1826,41 +1855,52 @@ class Impl final : public GpuFactor {
   void solve_levels_inv(double* dx, int nrhs) {
     const int64_t tv = P_.tvec_elems;
     const int nbk = SBW / NB;
+    const unsigned nch = static_cast<unsigned>((nrhs + R - 1) / R);  // right-hand-side chunks, one per workgroup
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_fwd_init, lp.fronts_r.count, 256, d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+                   nrhs);
       }
       for (const SbFwdStep& st : lp.sfwd) {
         if (st.chain.count > 0) {
-          hipLaunchKernelGGL(k_sfwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
-                             d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_t_, d_ys_, tv, nrhs);
+          hipLaunchKernelGGL(k_sfwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
+                             d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_t_, d_ys_, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
-        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_ys_, d_t_, tv, nrhs);
+        if (st.rows.count > 0) {
+          auto* kr = &k_sfwd_rows<T, R>;
+          hipLaunchKernelGGL(kr, dim3(grid(st.rows.count), nch), dim3(256), 0, s_, d_fd_, d_sbrow_ + st.rows.begin,
+                             d_arena_, d_ys_, d_t_, tv, nrhs);
+          HIPCHK(hipGetLastError());
+        }
       }
-      launch(k_fwd_finish_ys, lp.fronts_r.count, 256, d_fd_, fr, d_ys_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish_ys, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_ys_, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
       for (const SbBwdStep& st : lp.sbwd) {
-        launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
-               d_part_);
-        if (st.chain.count > 0) {
-          hipLaunchKernelGGL(k_sbwd_comb<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
-                             d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
+        if (st.part.count > 0) {
+          auto* kp = &k_sbwd_part<T, R>;
+          hipLaunchKernelGGL(kp, dim3(grid(st.part.count), nch), dim3(256), 0, s_, d_fd_, d_sbpart_ + st.part.begin,
+                             d_arena_, d_t_, tv, nrhs, d_part_);
           HIPCHK(hipGetLastError());
-          hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk), dim3(256), 0, s_, d_fd_,
-                             d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, d_ys_, d_t_, tv, nrhs);
+        }
+        if (st.chain.count > 0) {
+          hipLaunchKernelGGL(k_sbwd_comb<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
+                             d_fd_, d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
+          HIPCHK(hipGetLastError());
+          hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
+                             d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, d_ys_, d_t_, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
       }
-      launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
     }
   }
 
@@ -1874,29 +1914,30 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_fwd_init, lp.fronts_r.count, 256, d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch(k_fwd_child, cr.count, 256, d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv, nrhs);
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+                   nrhs);
       }
       for (const SbFwdStep& st : lp.sfwd) {
         launch(k_sfwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
                d_t_, tv, nrhs);
         launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_t_, d_t_, tv, nrhs);
       }
-      launch(k_fwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch(k_bwd_init, lp.fronts_r.count, 256, d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
       for (const SbBwdStep& st : lp.sbwd) {
         launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
                d_part_);
         launch(k_sbwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
                d_part_, d_t_, tv, nrhs);
       }
-      launch(k_bwd_finish, lp.fronts_r.count, 256, d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
     }
   }
 
@@ -1944,6 +1985,7 @@ class Impl final : public GpuFactor {
   bool autotune_ = true, tuned_ = false;
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): 1 = k_gemm_agpr
   bool chain_solve_ = false;
+  std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (4 or 16)
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;
