@@ -1766,11 +1766,13 @@ class Impl final : public GpuFactor {
     HIPCHK(hipStreamSynchronize(s_));
   }
 
-  // Several right-hand sides go through the kernels in passes of 4 by
-  // default: a pass of 16 needs about 100 KB of LDS per workgroup (one per
-  // CU), which the many small tiles of the lower levels cannot afford.
+  // Several right-hand sides go through the kernels in chunks of 4 by
+  // default, each chunk its own workgroups.  Measured on the 64^3 block with
+  // 16 RHS (three solves): chunks of 1 take 195 ms, of 4 102 ms, of 8 171 ms
+  // and of 16 232 ms -- a chunk of 16 needs about 100 KB of LDS per
+  // workgroup (one per CU), chunks of 1 read the factor 16 times.
   void solve_eager(double* dx, int nrhs) {
-    if (nrhs == 1)
+    if (nrhs == 1 || rhs_block_ == 1)
       solve_levels<1>(dx, nrhs);
     else if (nrhs <= 4 || rhs_block_ != 16)
       solve_levels<4>(dx, nrhs);
@@ -1986,7 +1988,7 @@ class Impl final : public GpuFactor {
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): 1 = k_gemm_agpr
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
-  int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (4 or 16)
+  int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (1, 4 or 16)
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;
   double* d_xinv_ = nullptr;
