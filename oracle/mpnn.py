@@ -27,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
-H, F = 16, 10
+H, F = 32, 10
 OMEGA = 2.0 / 3.0
 NPARAMS = H * F + H + H * 3 * H + H + 9 * H + 9
 

@@ -2,7 +2,7 @@
 // message-passing network maps the matrix's local blocks to a symmetric block
 // matrix M on the matrix's own block pattern; the smoother is x += M (b - A x).
 //
-// Network (FP64, H = 16 hidden units, F = 10 edge features), for every block
+// Network (FP64, H = 32 hidden units, F = 10 edge features), for every block
 // e = (i, j) of the pattern (including the diagonal block):
 //   s_i   = sqrt(3 / trace(A_ii))                       (node scale)
 //   x_e   = [s_i s_j vec(A_ij) (9, row-major), [i == j]] (edge features)
@@ -24,7 +24,7 @@
 namespace rf {
 
 constexpr int MPNN_F = 10;
-constexpr int MPNN_H = 16;
+constexpr int MPNN_H = 32;
 constexpr double MPNN_OMEGA = 2.0 / 3.0;
 constexpr int64_t MPNN_NPARAMS = MPNN_H * MPNN_F + MPNN_H + MPNN_H * 3 * MPNN_H + MPNN_H + 9 * MPNN_H + 9;
 

@@ -142,21 +142,23 @@ __global__ void k_mg_l2(int64_t nnzb, const double* __restrict__ prm, const int3
   const int64_t e = static_cast<int64_t>(blockIdx.x) * TB + threadIdx.x;
   if (e >= nnzb) return;
   const int32_t i = row[e], j = idx[e];
-  double z[3 * H];
-#pragma unroll
-  for (int a = 0; a < H; ++a) {
-    z[a] = relu(pre1[H * e + a]);
-    z[H + a] = node[H * i + a];
-    z[2 * H + a] = node[H * j + a];
-  }
+  // h = W2 [m_e; n_i; n_j] + b2, the input streamed entry by entry (each
+  // output still sums b = 0 .. 3H-1 in order), so only h stays in registers
   double h[H];
 #pragma unroll
-  for (int a = 0; a < H; ++a) {
-    double v = b2[a];
+  for (int a = 0; a < H; ++a) h[a] = b2[a];
+  for (int part = 0; part < 3; ++part) {
+    const double* src = part == 0 ? pre1 + H * e : node + H * static_cast<int64_t>(part == 1 ? i : j);
+    for (int b = 0; b < H; ++b) {
+      const double zb = part == 0 ? relu(src[b]) : src[b];
 #pragma unroll
-    for (int b = 0; b < 3 * H; ++b) v += W2[a * 3 * H + b] * z[b];
-    pre2[H * e + a] = v;
-    h[a] = relu(v);
+      for (int a = 0; a < H; ++a) h[a] += W2[a * 3 * H + part * H + b] * zb;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < H; ++a) {
+    pre2[H * e + a] = h[a];
+    h[a] = relu(h[a]);
   }
 #pragma unroll
   for (int q = 0; q < 9; ++q) {

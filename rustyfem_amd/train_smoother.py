@@ -22,7 +22,8 @@ import time
 
 import numpy as np
 
-NPARAMS_LAYOUT = (16 * 10, 16, 16 * 48, 16, 9 * 16, 9)
+H = 32  # hidden units (csrc/mpnn.hpp)
+NPARAMS_LAYOUT = (H * 10, H, H * 3 * H, H, 9 * H, 9)
 
 
 def lower_of(p):

@@ -59,4 +59,4 @@ def test_loss_matches_dense_evaluation():
         e = s @ (s @ e0[:, v])
         ratios.append((e @ a @ e) / (e0[:, v] @ a @ e0[:, v]))
     assert np.isclose(mpnn.loss(prm, ptr, idx, blocks, e0, nu=2), np.mean(ratios), rtol=1e-12)
-    assert mpnn.NPARAMS == 1113
+    assert mpnn.NPARAMS == 3753
