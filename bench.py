@@ -115,6 +115,27 @@ def lower_of(p):
     return low.indptr.astype(np.int64), low.indices.astype(np.int64), low.data.copy()
 
 
+def amg_cpu_baseline(p, n_full, tol, label):
+    """The oracle's AMG-PCG (the reference's CPU algorithm: smoothed aggregation,
+    Gauss-Seidel V-cycle, src/solver/iterative*.rs) on a smaller problem of the
+    same family, setup + solve timed on the host threads and scaled by DOF
+    count to the full problem (AMG iterations grow slowly with n, so the
+    scaled figure is a lower bound)."""
+    import oracle
+    from rustyfem_amd import meshes
+    _, coords, ndof, _, _ = meshes.node_graph(p)
+    cp, ri, v = lower_of(p)
+    b = p.rhs[:, 0].copy()
+    t0 = time.perf_counter()
+    x, rep = oracle.solve_lower(p.n, cp, ri, v, b, "amg", coords, ndof, tol=tol)
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: oracle AMG-PCG on {label} (n={p.n}) {dt:.2f}s, {rep['iterations']} iterations")
+    return {"value": dt * n_full / p.n, "unit": "s", "cores": int(oracle.threads()), "kind": "port",
+            "sample": f"{label} (n={p.n}): oracle AMG-PCG setup + solve to {tol:g} in {dt:.2f} s, "
+                      f"{rep['iterations']} iterations, scaled by DOF count to n={n_full}",
+            "sample_s": dt, "sample_iterations": rep["iterations"], "sample_converged": rep["converged"]}
+
+
 def run_amg(args):
     """The heterogeneous-stiffness block (C3D8 B-bar, lognormal E with log-std 1.5,
     correlation length 0.1, contrast clipped to 1e3, clamped base, body load):
@@ -175,6 +196,9 @@ def run_amg(args):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                         "frac": achieved / PEAK["hbm_gbs"], "traffic": None,
                         "kernel": "k_spmv<double,3,G> (BSR 3x3 fine-matrix product, FP64)"}}
+    if args.cpu_sample_iter > 0:
+        q = meshes.hetero_block(args.cpu_sample_iter, seed=4)
+        out["cpu_baseline"] = amg_cpu_baseline(q, p.n, tol, f"the heterogeneous block at {args.cpu_sample_iter}^3")
     print(json.dumps(out), flush=True)
 
 
@@ -262,6 +286,9 @@ def run_learned(args):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                         "frac": achieved / PEAK["hbm_gbs"], "traffic": None,
                         "kernel": "k_spmv<double,3,G> (BSR 3x3 fine-matrix product, FP64)"}}
+    if args.cpu_sample_iter > 0:
+        q = meshes.tet_block(args.cpu_sample_iter, seed=42)
+        out["cpu_baseline"] = amg_cpu_baseline(q, p.n, tol, f"the C3D4 block at {args.cpu_sample_iter}^3 cubes")
     print(json.dumps(out), flush=True)
 
 
@@ -428,6 +455,8 @@ def main():
     ap.add_argument("--size", type=int, default=64, help="elements per cube edge (64 = the 811,200-DOF block)")
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="cube edge of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample-iter", type=int, default=48,
+                    help="cube edge of the CPU-baseline sample of the AMG workloads (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned", "ic0"], default="direct",
                     help="direct: the multifrontal block (default); amg: the heterogeneous block with AMG-Krylov; "
