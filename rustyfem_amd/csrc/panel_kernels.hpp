@@ -87,6 +87,75 @@ __device__ __forceinline__ void block_ldlt(const double (*A)[NB + 1], double (*L
   for (int i = 0; i < NB; ++i) V[lane][i] = (lane < jb && i < jb && i >= lane) ? x[i] : 0.0;
 }
 
+// block_ldlt with the inverse pipelined behind the factorization, called by
+// every wave of the workgroup: in iteration j wave 0 runs elimination step j
+// and publishes column j of L to Ls, while wave 1 computes row j of L^-1
+// (row j of L is final after step j - 1); one workgroup barrier per
+// iteration.  The two waves keep their row / column in the same register
+// array.  The arithmetic, and so every result, is that of block_ldlt.
+__device__ __forceinline__ void block_ldlt_pipe(const double (*A)[NB + 1], double (*Ls)[NB + 2], double (*V)[NB + 2],
+                                                double* cbuf, double* dout, int jb, double orig, int64_t eq0,
+                                                unsigned long long* err) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  double r[NB];  // wave 0: row `lane` of the block; wave 1: column `lane` of L^-1
+  if (wave == 0) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) r[q] = A[q][lane];
+  } else {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) r[q] = q == lane ? 1.0 : 0.0;
+  }
+  __syncthreads();  // A has been read: Ls shares its bytes
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (wave == 0) {
+      double lj = 0.0;
+      if (j < jb) {
+        cbuf[lane] = r[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double w[NB];
+#pragma unroll
+        for (int q = j; q < NB; ++q) w[q] = cbuf[q];
+        __builtin_amdgcn_sched_barrier(0);
+        const double l = r[j] / w[j];
+#pragma unroll
+        for (int q = j + 1; q < NB; ++q) r[q] = fma(-l, w[q], r[q]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane > j) r[j] = l;
+        lj = l;
+      }
+      Ls[lane][j] = (j < jb && lane > j && lane < jb) ? lj : 0.0;
+    } else if (wave == 1) {
+      if (j >= 1 && j < jb) {
+        double w[NB];
+#pragma unroll
+        for (int p = 0; p < j; ++p) w[p] = Ls[j][p];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int p = 0; p < j; ++p) r[j] = fma(-w[p], r[p], r[j]);
+      }
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+    double dl = 0.0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q == lane) dl = r[q];
+    if (err != nullptr && lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
+      atomicMin(err, static_cast<unsigned long long>(eq0 + lane));
+    dout[lane] = lane < jb ? dl : 0.0;
+  } else if (wave == 1) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) V[lane][i] = (lane < jb && i < jb && i >= lane) ? r[i] : 0.0;
+  }
+}
+
 // Columns of the left-looking update staged in LDS per pass: the update of a
 // pivot block reads at most NBO - NB = 192 earlier columns of its outer
 // block; one pass in FP32 (50 KB of LDS), two in FP64.
@@ -201,9 +270,7 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
       Ad[q][i] = (i < jb && q < jb) ? static_cast<double>(fb[bj][r]) - cb[bj][r] : 0.0;
     }
   __syncthreads();
-  if (wave == 0) {
-    block_ldlt(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, err);
-  }
+  block_ldlt_pipe(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, err);
   __syncthreads();
   double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
