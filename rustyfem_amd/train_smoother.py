@@ -103,7 +103,7 @@ def evaluate(params, systems=16, nx=10, first_seed=900, nvec=8, nu=2, seed=1):
     from rustyfem_amd import ffi, meshes
     rng = np.random.default_rng(seed)
     start = initial_params(0)
-    start[-(9 * 16 + 9):] = 0.0
+    start[-(NPARAMS_LAYOUT[4] + NPARAMS_LAYOUT[5]):] = 0.0
     lt, l0 = [], []
     for s in range(first_seed, first_seed + systems):
         p = meshes.tet_block(nx, seed=s)
