@@ -358,6 +358,25 @@ bool batch_numeric(BatchHandle* h, const double* const* data, int64_t* err) {
   return false;
 }
 
+// entries of rustyfem_stats / rustyfem_batch_stats that come from the
+// factorization's timings (the layout is documented in rustyfem.h)
+constexpr size_t NSTATS = 21;
+void fill_timings(const rf::Timings& t, double* v) {
+  v[0] = t.factor_ms;
+  v[1] = t.solve_ms;
+  v[2] = t.refine_ms;
+  v[3] = t.refine_steps;
+  v[4] = t.rel_residual;
+  v[10] = t.gemm_ms;
+  v[11] = t.gemm_launches;
+  v[12] = t.gemm_flops;
+  v[13] = t.tune_ms;
+  v[14] = t.tuned_launches;
+  v[15] = t.agpr_launches;
+  for (int i = 0; i < 4; ++i) v[16 + i] = t.refine_history[i];
+  v[20] = t.ilv_launches;
+}
+
 }  // namespace
 
 extern "C" {
@@ -486,25 +505,14 @@ void rustyfem_batch_info(const BatchHandle* h, double* out) {
 }
 
 void rustyfem_batch_stats(const BatchHandle* h, double* out, size_t len) {
-  double v[16] = {0};
-  const rf::Timings& t = h->gpu->timings();
-  v[0] = t.factor_ms;
-  v[1] = t.solve_ms;
-  v[2] = t.refine_ms;
-  v[3] = t.refine_steps;
-  v[4] = t.rel_residual;
+  double v[NSTATS] = {0};
+  fill_timings(h->gpu->timings(), v);
   v[5] = h->gpu->flops();
   v[6] = static_cast<double>(h->gpu->device_bytes());
   v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
   v[8] = static_cast<double>(h->nfronts);
   v[9] = static_cast<double>(h->nlevels);
-  v[10] = t.gemm_ms;
-  v[11] = t.gemm_launches;
-  v[12] = t.gemm_flops;
-  v[13] = t.tune_ms;
-  v[14] = t.tuned_launches;
-  v[15] = t.agpr_launches;
-  for (size_t i = 0; i < len && i < 20; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < NSTATS; ++i) out[i] = v[i];
 }
 
 void rustyfem_free_batch(BatchHandle* h) { delete h; }
@@ -608,28 +616,16 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
 }
 
 void rustyfem_stats(const Handle* h, double* out, size_t len) {
-  double v[20] = {0};
+  double v[NSTATS] = {0};
   if (h->gpu) {
-    const rf::Timings& t = h->gpu->timings();
-    v[0] = t.factor_ms;
-    v[1] = t.solve_ms;
-    v[2] = t.refine_ms;
-    v[3] = t.refine_steps;
-    v[4] = t.rel_residual;
+    fill_timings(h->gpu->timings(), v);
     v[5] = h->gpu->flops();
     v[6] = static_cast<double>(h->gpu->device_bytes());
-    v[10] = t.gemm_ms;
-    v[11] = t.gemm_launches;
-    v[12] = t.gemm_flops;
-    v[13] = t.tune_ms;
-    v[14] = t.tuned_launches;
-    v[15] = t.agpr_launches;
-    for (int i = 0; i < 4; ++i) v[16 + i] = t.refine_history[i];
   }
   v[7] = static_cast<double>(static_cast<int>(h->opt.prec));
   v[8] = static_cast<double>(h->nfronts);
   v[9] = static_cast<double>(h->nlevels);
-  for (size_t i = 0; i < len && i < 20; ++i) out[i] = v[i];
+  for (size_t i = 0; i < len && i < NSTATS; ++i) out[i] = v[i];
 }
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }

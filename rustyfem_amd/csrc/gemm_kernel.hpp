@@ -131,6 +131,144 @@ __device__ __forceinline__ void gemm_tile(const FrontDesc* __restrict__ fd, cons
   }
 }
 
+// gemm_tile with K tiles twice as wide (64 columns in FP32, 32 in FP64: half
+// the barriers per product and twice the time for the next tile's loads to
+// land) and cheaper loads spread among the products.  A column's address is
+// the K tile's first column (one 64-bit base per tile) plus a 32-bit offset,
+// one scalar multiply-add per column instead of a 64-bit product, and the
+// lane's row is a 32-bit vector offset on it.  The next K tile's loads sit in
+// the same basic block as the current tile's MFMAs (the last K tile is peeled
+// off), and a scheduling pattern interleaves them with the first products.
+// Every accumulator receives the same MFMA sequence as in gemm_tile (k in
+// increasing order, four k per 16-byte LDS read in FP32, two in FP64): both
+// give identical results.
+template <typename T>
+__device__ __forceinline__ void gemm_tile_ilv(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                              T* __restrict__ arena, const double* __restrict__ dpiv) {
+  using M = Mfma<T>;
+  using V4 = typename M::acc_t;
+  using L16 = typename M::lds16_t;
+  constexpr int E = 16 / static_cast<int>(sizeof(T));
+  constexpr int KT = 2 * M::KT;
+  constexpr int KP = KT + 16 / static_cast<int>(sizeof(T));
+  constexpr int NQ = KT / 8;
+  const GemmTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  __shared__ __attribute__((aligned(16))) T As[GT_M][KP];
+  __shared__ __attribute__((aligned(16))) T Bs[GT_M][KP];
+  __shared__ __attribute__((aligned(16))) T Ds[KT];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1, kg = wave >> 1;
+  const int li = tid & 127;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  const int arow = min(t.r0 + li, d.m - 1);
+  const int brow = min(t.c0 + li, t.cend - 1);
+  // a front holds m^2 entries, so KT columns of it span far less than 4 GiB
+  const uint32_t mb = static_cast<uint32_t>(m * static_cast<int64_t>(sizeof(T)));
+  const uint32_t aoff = static_cast<uint32_t>(arow) * static_cast<uint32_t>(sizeof(T));
+  const uint32_t boff = static_cast<uint32_t>(brow) * static_cast<uint32_t>(sizeof(T));
+  const char* Fb = reinterpret_cast<const char*>(F);
+  V4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = V4{0, 0, 0, 0};
+  T ra[4 * NQ], rb[4 * NQ];
+  double rd = 0.0;
+  // thread (li, kg) loads row li of both tiles at k = 4 (kg + 2 q) + j; every
+  // thread loads a D entry (the store keeps those of threads < KT)
+  auto gload = [&](int kb) {
+    const char* base = Fb + static_cast<int64_t>(kb) * m * static_cast<int64_t>(sizeof(T));
+    const int lim = t.k1 - 1 - kb;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = min(4 * (kg + 2 * q) + j, lim);
+        const char* col = base + static_cast<uint32_t>(c) * mb;
+        ra[4 * q + j] = *reinterpret_cast<const T*>(col + aoff);
+        rb[4 * q + j] = *reinterpret_cast<const T*>(col + boff);
+      }
+    rd = dpiv[d.start + min(kb + (tid & (KT - 1)), t.k1 - 1)];
+  };
+  auto store = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int k4 = 4 * (kg + 2 * q);
+      lds_put4(&As[li][k4], ra[4 * q], ra[4 * q + 1], ra[4 * q + 2], ra[4 * q + 3]);
+      lds_put4(&Bs[li][k4], rb[4 * q], rb[4 * q + 1], rb[4 * q + 2], rb[4 * q + 3]);
+    }
+    if (tid < KT) Ds[tid] = kb + tid < t.k1 ? static_cast<T>(rd) : T(0);
+  };
+  const int g = lane >> 4, l16 = lane & 15;
+  auto products = [&]() {
+#pragma unroll
+    for (int h = 0; h < KT / 16; ++h)
+#pragma unroll
+      for (int p = 0; p < 4 / E; ++p) {
+        const int kk = 16 * h + 4 * g + E * p;
+        L16 av[4], bv[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] = *reinterpret_cast<const L16*>(&As[wm * 64 + bi * 16 + l16][kk]);
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) bv[bj] = *reinterpret_cast<const L16*>(&Bs[wn * 64 + bj * 16 + l16][kk]);
+        const L16 dv = *reinterpret_cast<const L16*>(&Ds[kk]);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] *= dv;
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(bv[bj][s], av[bi][s], acc[bi][bj]);
+      }
+  };
+  gload(t.k0);
+  int kb = t.k0;
+  for (; kb + KT < t.k1; kb += KT) {
+    __syncthreads();
+    store(kb);
+    __syncthreads();
+    gload(kb + KT);
+    products();
+    // one load (and the two scalar instructions of its address) after each
+    // of the first 8 NQ products
+#pragma unroll
+    for (int i = 0; i < 8 * NQ; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x004, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+  }
+  __syncthreads();
+  store(kb);
+  __syncthreads();
+  products();
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int row = t.r0 + wm * 64 + bi * 16 + l16;
+    const int rowc = min(row, d.m - 1);
+    T cv[4][4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int colc = min(t.c0 + wn * 64 + bj * 16 + M::orow(lane, r), t.cend - 1);
+        cv[bj][r] = F[rowc + static_cast<int64_t>(colc) * m];
+      }
+    if (row >= d.m) continue;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = t.c0 + wn * 64 + bj * 16 + M::orow(lane, r);
+        if (col < t.cend) F[row + static_cast<int64_t>(col) * m] = cv[bj][r] - acc[bi][bj][r];
+      }
+  }
+}
+
 // The same tile under two register budgets: k_gemm asks for two waves per
 // SIMD (accumulators in VGPRs; FP32 reaches three), k_gemm_agpr leaves the
 // accumulators in AGPRs (two waves per SIMD in FP32, one in FP64).  Which is
@@ -147,6 +285,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_agpr(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
                                                    T* __restrict__ arena, const double* __restrict__ dpiv) {
   gemm_tile<T>(fd, tiles, arena, dpiv);
+}
+
+// gemm_tile_ilv, two waves per SIMD
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_ilv(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                                  T* __restrict__ arena, const double* __restrict__ dpiv) {
+  gemm_tile_ilv<T>(fd, tiles, arena, dpiv);
 }
 
 }  // namespace rf

@@ -1378,12 +1378,13 @@ class Impl final : public GpuFactor {
       chain_solve_ = c && *c && *c != '0';
       const char* rb = std::getenv("RUSTYFEM_SOLVE_RHS_BLOCK");
       if (rb && *rb) rhs_block_ = std::atoi(rb);
-      // RUSTYFEM_AUTOTUNE: unset or 1 = time both GEMM kernels per launch,
-      // 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere
+      // RUSTYFEM_AUTOTUNE: unset or 1 = time the three GEMM kernels per
+      // launch, 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere,
+      // "ilv" = k_gemm_ilv everywhere
       const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
-      const bool force_agpr = a && std::strcmp(a, "agpr") == 0;
-      autotune_ = !(a && *a == '0') && !force_agpr;
-      gemm_pick_.assign(P_.gemm.size(), force_agpr ? 1 : 0);
+      const int8_t force = a && std::strcmp(a, "agpr") == 0 ? 1 : a && std::strcmp(a, "ilv") == 0 ? 2 : 0;
+      autotune_ = !(a && *a == '0') && force == 0;
+      gemm_pick_.assign(P_.gemm.size(), force);
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -1616,10 +1617,7 @@ class Impl final : public GpuFactor {
       HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
     }
     if (ph.kind != PhaseKind::Gemm128) throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
-    if (gemm_pick_[r.begin])
-      launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
-    else
-      launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    launch_gemm(gemm_pick_[r.begin], r);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
@@ -1627,9 +1625,20 @@ class Impl final : public GpuFactor {
     }
   }
 
-  // Autotuning of the GEMM register layout: every GEMM launch of the plan runs
-  // both variants (one warm-up, then three timed launches each, the minimum
-  // kept) and keeps k_gemm_agpr where it is at least 3% faster.  The launches
+  // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv
+  void launch_gemm(int variant, const Range& r) {
+    if (variant == 1)
+      launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else if (variant == 2)
+      launch(k_gemm_ilv<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else
+      launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+  }
+
+  // Autotuning of the GEMM kernel: every GEMM launch of the plan runs the
+  // three variants (one warm-up, then three timed launches each, the minimum
+  // kept); a variant replaces the current choice (k_gemm first) only where it
+  // is at least 3% faster.  All three give identical results.  The launches
   // work on the arena before the first factorization assembles anything in
   // it; the factorization clears the arena and every level assembles its
   // fronts before its own GEMMs, so
@@ -1639,46 +1648,46 @@ class Impl final : public GpuFactor {
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
         if (p.kind == PhaseKind::Gemm128 && p.r.count > 0) ph.push_back(&p);
-    std::vector<hipEvent_t> ev(ph.size() * 8);
+    constexpr int NV = 3;
+    std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(e0_, s_));
     for (size_t i = 0; i < ph.size(); ++i) {
       const Range& r = ph[i]->r;
-      for (int v = 0; v < 2; ++v) {
-        auto run = [&]() {
-          if (v)
-            launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
-          else
-            launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
-        };
-        run();
+      for (int v = 0; v < NV; ++v) {
+        launch_gemm(v, r);
         for (int k = 0; k < 4; ++k) {
-          HIPCHK(hipEventRecord(ev[8 * i + 4 * v + k], s_));
-          if (k < 3) run();
+          HIPCHK(hipEventRecord(ev[4 * (NV * i + v) + k], s_));
+          if (k < 3) launch_gemm(v, r);
         }
       }
     }
     HIPCHK(hipEventRecord(e1_, s_));
     HIPCHK(hipStreamSynchronize(s_));
-    int64_t alt = 0;
+    int64_t count[NV] = {0, 0, 0};
     for (size_t i = 0; i < ph.size(); ++i) {
-      float best[2] = {1e30f, 1e30f};
-      for (int v = 0; v < 2; ++v)
+      float best[NV];
+      for (int v = 0; v < NV; ++v) {
+        best[v] = 1e30f;
         for (int k = 0; k < 3; ++k) {
           float t = 0;
-          HIPCHK(hipEventElapsedTime(&t, ev[8 * i + 4 * v + k], ev[8 * i + 4 * v + k + 1]));
+          HIPCHK(hipEventElapsedTime(&t, ev[4 * (NV * i + v) + k], ev[4 * (NV * i + v) + k + 1]));
           best[v] = std::min(best[v], t);
         }
-      const bool pick = best[1] < 0.97f * best[0];
-      gemm_pick_[ph[i]->r.begin] = pick ? 1 : 0;
-      alt += pick ? 1 : 0;
+      }
+      int pick = 0;
+      for (int v = 1; v < NV; ++v)
+        if (best[v] < 0.97f * best[pick]) pick = v;
+      gemm_pick_[ph[i]->r.begin] = static_cast<int8_t>(pick);
+      ++count[pick];
     }
     for (auto& e : ev) HIPCHK(hipEventDestroy(e));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
     tm_.tune_ms = ms;
     tm_.tuned_launches = static_cast<double>(ph.size());
-    tm_.agpr_launches = static_cast<double>(alt);
+    tm_.agpr_launches = static_cast<double>(count[1]);
+    tm_.ilv_launches = static_cast<double>(count[2]);
     tuned_ = true;
   }
 
@@ -1985,7 +1994,7 @@ class Impl final : public GpuFactor {
   int nseg_ = 0, segsq_cap_ = 0;
   bool legacy_panel_ = false;
   bool autotune_ = true, tuned_ = false;
-  std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): 1 = k_gemm_agpr
+  std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): the launch_gemm variant
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (1, 4 or 16)

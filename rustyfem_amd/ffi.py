@@ -167,6 +167,21 @@ def _arrays(p):
     return keep, [int(p.n), int(p.nn)] + [a.ctypes.data for a in keep]
 
 
+_STATS_KEYS = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
+               "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
+               "tuned_launches", "agpr_launches"]
+
+
+def _stats_dict(fn, h):
+    """rustyfem_stats / rustyfem_batch_stats as a dict (21 entries, rustyfem.h)."""
+    o = np.zeros(21)
+    fn(h, o.ctypes.data, 21)
+    d = dict(zip(_STATS_KEYS, o.tolist()))
+    d["refine_history"] = [float(v) for v in o[16:20] if v >= 0]
+    d["ilv_launches"] = float(o[20])
+    return d
+
+
 class Handle:
     """A factorization held by the library (ffi/src/lib.rs:44-55)."""
 
@@ -273,14 +288,7 @@ class Handle:
         return o
 
     def stats(self):
-        o = np.zeros(20)
-        lib().rustyfem_stats(self._h, o.ctypes.data, 20)
-        keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
-                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
-                "tuned_launches", "agpr_launches"]
-        d = dict(zip(keys, o.tolist()))
-        d["refine_history"] = [float(v) for v in o[16:20] if v >= 0]
-        return d
+        return _stats_dict(lib().rustyfem_stats, self._h)
 
 
 KIND = {"amg": 0, "ic0": 1, "jacobi": 2}
@@ -479,12 +487,7 @@ class Batch:
         return {"problems": int(o[0]), "n": int(o[1]), "fronts": int(o[2]), "levels": int(o[3]), "work": o[4]}
 
     def stats(self):
-        o = np.zeros(16)
-        lib().rustyfem_batch_stats(self._h, o.ctypes.data, 16)
-        keys = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residual", "flops", "device_bytes",
-                "precision", "fronts", "levels", "gemm_ms", "gemm_launches", "gemm_flops", "tune_ms",
-                "tuned_launches", "agpr_launches"]
-        return dict(zip(keys, o.tolist()))
+        return _stats_dict(lib().rustyfem_batch_stats, self._h)
 
     def order(self, i):
         o = np.zeros(max(int(self.ns[i]), 1), dtype=np.int64)
