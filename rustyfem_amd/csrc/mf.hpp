@@ -85,7 +85,10 @@ struct Range { int64_t begin = 0, count = 0; };
 // Panel: the fused inner step (every tile factors the pivot block itself);
 // PanelDiag / PanelRows: the same step as two launches (pivot blocks, then
 // the row tiles below them)
-enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel, PanelDiag, PanelRows };
+// GemmRest: the part of an outer block's right-looking update that the next
+// outer block's panel does not read (pivot columns from J1 + NBO on); it may
+// run beside the panel phases that follow it, up to the next non-panel phase.
+enum class PhaseKind : int32_t { Gemm64, Gemm128, Diag, Trsm, Panel, PanelDiag, PanelRows, GemmRest };
 
 struct Phase {
   PhaseKind kind;

@@ -1340,6 +1340,13 @@ class Impl final : public GpuFactor {
  public:
   Impl(Plan plan, const std::vector<int64_t>& colptr, const std::vector<int64_t>& rowidx) : P_(std::move(plan)) {
     HIPCHK(hipStreamCreate(&s_));
+    HIPCHK(hipStreamCreate(&s2_));
+    HIPCHK(hipEventCreateWithFlags(&ev_look_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming));
+    {
+      const char* la = std::getenv("RUSTYFEM_NO_LOOKAHEAD");
+      lookahead_ = !(la && *la && *la != '0');
+    }
     const char* ng = std::getenv("RUSTYFEM_NO_GRAPH");
     use_graphs_ = !(ng && *ng && *ng != '0');
     HIPCHK(hipEventCreate(&e0_));
@@ -1449,6 +1456,7 @@ class Impl final : public GpuFactor {
   }
 
   ~Impl() override {
+    (void)hipStreamSynchronize(s2_);
     (void)hipStreamSynchronize(s_);
     drop_graphs();
     void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
@@ -1459,6 +1467,9 @@ class Impl final : public GpuFactor {
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     (void)hipEventDestroy(e0_);
     (void)hipEventDestroy(e1_);
+    (void)hipEventDestroy(ev_look_);
+    (void)hipEventDestroy(ev_rest_);
+    (void)hipStreamDestroy(s2_);
     (void)hipStreamDestroy(s_);
   }
 
@@ -1474,14 +1485,31 @@ class Impl final : public GpuFactor {
     tm_.gemm_flops = 0;
     launch(k_zero<T>, static_cast<int64_t>(P_.zero.size()), 256, d_fd_, d_zero_, d_arena_);
     for (const LevelPlan& lp : P_.levels) {
+      join_rest();
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
       for (const Phase& ph : lp.phases) {
+        // only the pivot-block phases may run beside the rest of an update
+        if (ph.kind != PhaseKind::PanelDiag && ph.kind != PhaseKind::PanelRows && ph.kind != PhaseKind::Panel)
+          join_rest();
         switch (ph.kind) {
           case PhaseKind::Gemm64:
           case PhaseKind::Gemm128:
-            gemm(ph);
+            gemm(ph, s_);
+            break;
+          case PhaseKind::GemmRest:
+            if (lookahead_ && ph.r.count > 0) {
+              // on the second stream, after everything issued so far on the
+              // main one; the next panel phases go on beside it
+              HIPCHK(hipEventRecord(ev_look_, s_));
+              HIPCHK(hipStreamWaitEvent(s2_, ev_look_, 0));
+              gemm(ph, s2_);
+              HIPCHK(hipEventRecord(ev_rest_, s2_));
+              rest_pending_ = true;
+            } else {
+              gemm(ph, s_);
+            }
             break;
           case PhaseKind::Diag:
             launch(k_diag<T>, ph.r.count, 256, d_fd_, d_diag_, ph.r.begin, d_arena_, d_dpiv_, d_adiag_, d_vbuf_,
@@ -1512,6 +1540,7 @@ class Impl final : public GpuFactor {
         }
       }
     }
+    join_rest();
     // explicit inverses of the super-blocks for the solves
     if (!legacy_solve_ && !chain_solve_)
       launch(k_sbinv<T>, static_cast<int64_t>(P_.sbinv.size()), 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
@@ -1586,9 +1615,21 @@ class Impl final : public GpuFactor {
  private:
   template <typename K, typename... A>
   void launch(K kernel, int64_t blocks, int threads, A... args) {
+    launch_on(s_, kernel, blocks, threads, args...);
+  }
+
+  template <typename K, typename... A>
+  void launch_on(hipStream_t st, K kernel, int64_t blocks, int threads, A... args) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(kernel, dim3(grid(blocks)), dim3(threads), 0, s_, args...);
+    hipLaunchKernelGGL(kernel, dim3(grid(blocks)), dim3(threads), 0, st, args...);
     HIPCHK(hipGetLastError());
+  }
+
+  // the main stream waits for the rest of the last outer block's update
+  void join_rest() {
+    if (!rest_pending_) return;
+    HIPCHK(hipStreamWaitEvent(s_, ev_rest_, 0));
+    rest_pending_ = false;
   }
 
   // a per-front vector kernel over `fronts` fronts, nrhs right-hand sides and
@@ -1601,8 +1642,8 @@ class Impl final : public GpuFactor {
     HIPCHK(hipGetLastError());
   }
 
-  // one launch of the Schur-update GEMM over a tile range
-  void gemm(const Phase& ph) {
+  // one launch of the Schur-update GEMM over a tile range, on stream st
+  void gemm(const Phase& ph, hipStream_t st) {
     const Range& r = ph.r;
     const double flops = ph.flops;
     if (r.count <= 0) return;
@@ -1614,25 +1655,26 @@ class Impl final : public GpuFactor {
           evs_.push_back(e);
         }
       }
-      HIPCHK(hipEventRecord(evs_[2 * nev_], s_));
+      HIPCHK(hipEventRecord(evs_[2 * nev_], st));
     }
-    if (ph.kind != PhaseKind::Gemm128) throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
-    launch_gemm(gemm_pick_[r.begin], r);
+    if (ph.kind != PhaseKind::Gemm128 && ph.kind != PhaseKind::GemmRest)
+      throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
+    launch_gemm(gemm_pick_[r.begin], r, st);
     tm_.gemm_flops += flops;
     if (ktime_) {
-      HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], s_));
+      HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], st));
       ++nev_;
     }
   }
 
   // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv
-  void launch_gemm(int variant, const Range& r) {
+  void launch_gemm(int variant, const Range& r, hipStream_t st) {
     if (variant == 1)
-      launch(k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+      launch_on(st, k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     else if (variant == 2)
-      launch(k_gemm_ilv<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+      launch_on(st, k_gemm_ilv<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     else
-      launch(k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+      launch_on(st, k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
   }
 
   // Autotuning of the GEMM kernel: every GEMM launch of the plan runs the
@@ -1647,7 +1689,7 @@ class Impl final : public GpuFactor {
     std::vector<const Phase*> ph;
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
-        if (p.kind == PhaseKind::Gemm128 && p.r.count > 0) ph.push_back(&p);
+        if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0) ph.push_back(&p);
     constexpr int NV = 3;
     std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -1655,10 +1697,10 @@ class Impl final : public GpuFactor {
     for (size_t i = 0; i < ph.size(); ++i) {
       const Range& r = ph[i]->r;
       for (int v = 0; v < NV; ++v) {
-        launch_gemm(v, r);
+        launch_gemm(v, r, s_);
         for (int k = 0; k < 4; ++k) {
           HIPCHK(hipEventRecord(ev[4 * (NV * i + v) + k], s_));
-          if (k < 3) launch_gemm(v, r);
+          if (k < 3) launch_gemm(v, r, s_);
         }
       }
     }
@@ -1955,6 +1997,12 @@ class Impl final : public GpuFactor {
   static constexpr int kRedBlocks = 1024;
   Plan P_;
   hipStream_t s_ = nullptr;
+  // the rest of an outer block's update (PhaseKind::GemmRest) runs on s2_
+  // beside the next outer block's panel on s_; ev_look_ orders it after the
+  // work before it, ev_rest_ orders the next non-panel phase after it
+  hipStream_t s2_ = nullptr;
+  hipEvent_t ev_look_ = nullptr, ev_rest_ = nullptr;
+  bool lookahead_ = true, rest_pending_ = false;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   int64_t n_ = 0, nlower_ = 0, nfull_ = 0;
   size_t bytes_ = 0;

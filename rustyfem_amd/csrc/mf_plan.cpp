@@ -178,20 +178,26 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
         lp.solve.push_back(SolveStep{take(db, static_cast<int64_t>(P.diag.size())),
                                      take(sb, static_cast<int64_t>(P.srow.size()))});
       }
-      // right-looking update of the remaining pivot columns [J1, k), K = [J0, J1)
+      // right-looking update of the remaining pivot columns [J1, k), K = [J0, J1):
+      // first the columns [J1, J1 + NBO) of the next outer block (its panel
+      // waits for them), then the rest, which may run beside that panel
       const int32_t J1 = J0 + NBO;
-      int64_t b = static_cast<int64_t>(P.gemm.size());
-      double fl = 0;
-      for (int32_t f : lp.fronts) {
-        const FrontDesc& d = P.fd[f];
-        if (d.k <= J1) continue;
-        for (int32_t c0 = J1; c0 < d.k; c0 += 128)
-          for (int32_t r0 = J1; r0 < d.m; r0 += GT_M)
-            if (c0 <= r0 + GT_M - 1) P.gemm.push_back({f, r0, c0, d.k, J0, J1});
-        const double w = d.k - J1, below = d.m - d.k;
-        fl += 2.0 * NBO * (w * (w + 1.0) / 2.0 + below * w);
+      // flops of the update of columns [a, b) (rows c..m of column c)
+      auto colflops = [](double a, double b, double m) { return 2.0 * NBO * ((b - a) * m - (a + b - 1.0) * (b - a) / 2.0); };
+      for (int part = 0; part < 2; ++part) {
+        const int64_t b = static_cast<int64_t>(P.gemm.size());
+        double fl = 0;
+        for (int32_t f : lp.fronts) {
+          const FrontDesc& d = P.fd[f];
+          if (d.k <= J1) continue;
+          const int32_t cb = part == 0 ? J1 : std::min(J1 + NBO, d.k), ce = part == 0 ? std::min(J1 + NBO, d.k) : d.k;
+          for (int32_t c0 = cb; c0 < ce; c0 += 128)
+            for (int32_t r0 = J1; r0 < d.m; r0 += GT_M)
+              if (c0 <= r0 + GT_M - 1) P.gemm.push_back({f, r0, c0, d.k, J0, J1});
+          fl += colflops(cb, ce, d.m);
+        }
+        phase(part == 0 ? PhaseKind::Gemm128 : PhaseKind::GemmRest, b, static_cast<int64_t>(P.gemm.size()), fl);
       }
-      phase(PhaseKind::Gemm128, b, static_cast<int64_t>(P.gemm.size()), fl);
     }
     // backward solve of the level
     {

@@ -1,8 +1,11 @@
 // Static hazard check of a factorization plan: inside one launch, no tile may
 // write an arena element that another tile of the same launch reads or
-// writes.  Footprints are rectangles (front, rows, columns) of the column-major
-// fronts; the extend-add footprints are sets of parent entries reached
-// through the child->parent row map.
+// writes; and the rest of an outer block's update (a GemmRest launch, which
+// runs on a second stream) may not write what the panel launches running
+// beside it read or write, nor read what they write.  Footprints are
+// rectangles (front, rows, columns) of the column-major fronts; the
+// extend-add footprints are sets of parent entries reached through the
+// child->parent row map.
 #include <algorithm>
 #include <cstdio>
 #include <string>
@@ -49,6 +52,16 @@ std::string check_rects(std::vector<Rect>& v, const char* what) {
   }
   return "";
 }
+
+// rectangles of two launches that run at the same time
+std::string check_cross(const std::vector<Rect>& a, const std::vector<Rect>& b, const char* what) {
+  for (const Rect& x : a)
+    for (const Rect& y : b)
+      if ((x.write || y.write) && overlap(x, y)) return describe(what, x, y);
+  return "";
+}
+
+bool is_panel(PhaseKind k) { return k == PhaseKind::Panel || k == PhaseKind::PanelDiag || k == PhaseKind::PanelRows; }
 
 }  // namespace
 
@@ -179,14 +192,14 @@ std::string check_plan(const Plan& P, int plant) {
           return buf;
         }
     }
-    for (const Phase& ph : lp.phases) {
+    // footprints of one launch
+    auto rects_of = [&](const Phase& ph) {
       std::vector<Rect> v;
       if (ph.kind == PhaseKind::PanelDiag || ph.kind == PhaseKind::PanelRows) {
         for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
           const PanelTile& pt = P.panel[t];
           const int32_t j1 = pt.j0 + pt.jb, f = pt.front, m = P.fd[f].m, id = static_cast<int32_t>(t);
           const int32_t rend = std::min(pt.r0 + TRS, m);
-          if ((ph.kind == PhaseKind::PanelDiag) != (pt.r0 == pt.j0)) return "level " + std::to_string(L) + " panel tile kind";
           v.push_back({f, pt.j0, j1, pt.J0, pt.j0, false, id});  // Y
           if (ph.kind == PhaseKind::PanelDiag) {
             v.push_back({f, pt.j0, j1, pt.j0, j1, false, id});   // the pivot block (only read)
@@ -213,7 +226,7 @@ std::string check_plan(const Plan& P, int plant) {
             v.push_back({f, pt.r0, rend, pt.j0, j1, true, id});      // L_R
           }
         }
-      } else if (ph.kind == PhaseKind::Gemm64 || ph.kind == PhaseKind::Gemm128) {
+      } else if (ph.kind == PhaseKind::Gemm64 || ph.kind == PhaseKind::Gemm128 || ph.kind == PhaseKind::GemmRest) {
         const int tn = ph.kind == PhaseKind::Gemm64 ? 64 : 128;
         for (int64_t t = ph.r.begin; t < ph.r.begin + ph.r.count; ++t) {
           const GemmTile& g = P.gemm[t];
@@ -222,13 +235,27 @@ std::string check_plan(const Plan& P, int plant) {
           v.push_back({g.front, g.r0, rend, g.k0, g.k1, false, id});
           v.push_back({g.front, g.c0, cend, g.k0, g.k1, false, id});
           v.push_back({g.front, g.r0, rend, g.c0, cend, true, id});
+          // planted: the first tile of a rest update also writes the first
+          // column of the next outer block, which the panel beside it reads
+          if (plant == 2 && ph.kind == PhaseKind::GemmRest && t == ph.r.begin)
+            v.push_back({g.front, g.k1, m, g.k1, g.k1 + 1, true, id});
         }
-      } else {
-        continue;
       }
-      const bool panel = ph.kind == PhaseKind::Panel || ph.kind == PhaseKind::PanelDiag || ph.kind == PhaseKind::PanelRows;
+      return v;
+    };
+    for (size_t i = 0; i < lp.phases.size(); ++i) {
+      const Phase& ph = lp.phases[i];
+      std::vector<Rect> v = rects_of(ph);
+      if (v.empty()) continue;
+      const bool panel = is_panel(ph.kind);
       const std::string e = check_rects(v, panel ? "panel" : "gemm");
       if (!e.empty()) return "level " + std::to_string(L) + " " + e;
+      if (ph.kind != PhaseKind::GemmRest) continue;
+      const std::vector<Rect> rest = rects_of(ph);
+      for (size_t j = i + 1; j < lp.phases.size() && is_panel(lp.phases[j].kind); ++j) {
+        const std::string c = check_cross(rest, rects_of(lp.phases[j]), "lookahead");
+        if (!c.empty()) return "level " + std::to_string(L) + " " + c;
+      }
     }
   }
   return "";

@@ -107,3 +107,14 @@ def test_schedule_check_catches_planted_race(rf):
     diagonal tile while the row tiles of the same launch read it."""
     msg = rf.check_plan(meshes.hex_block(8), plant=1)
     assert msg.startswith("level") and "panel" in msg
+
+
+def test_lookahead_schedule_is_clean_and_checked(rf):
+    """The rest of each outer block's update runs on a second stream beside the
+    next outer block's pivot-block launches: on a block whose top fronts have
+    more than 512 pivots (so rest launches exist) the schedule is clean, and a
+    planted write of the next block's first column by the rest is caught."""
+    p = meshes.hex_block(20)
+    assert rf.check_plan(p) == ""
+    msg = rf.check_plan(p, plant=2)
+    assert msg.startswith("level") and "lookahead" in msg

@@ -211,3 +211,22 @@ def test_gemm_register_layouts_agree(rf, monkeypatch, precision):
     assert st["tuned_launches"] > 0
     assert 0 <= st["agpr_launches"] + st["ilv_launches"] <= st["tuned_launches"]
     assert h_v.stats()["tuned_launches"] == 0 and h_i.stats()["tuned_launches"] == 0
+
+
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_lookahead_gives_identical_factor(rf, monkeypatch, precision):
+    """The rest of each outer block's update runs on a second stream beside the
+    next outer block's pivot-block launches; on a block whose top fronts have
+    more than 512 pivots, the factor and the solutions are identical to those
+    of the single-stream schedule (RUSTYFEM_NO_LOOKAHEAD=1)."""
+    p = meshes.hex_block(20, nrhs=2, seed=9)
+    print(f"case lookahead hex20 n={p.n} precision={precision}", flush=True)
+    monkeypatch.setenv("RUSTYFEM_NO_LOOKAHEAD", "1")
+    h_s, _, d_s = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+    monkeypatch.delenv("RUSTYFEM_NO_LOOKAHEAD")
+    h_l, _, d_l = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+    assert np.array_equal(d_l, d_s)
+    assert np.array_equal(h_l.solve(p.rhs), h_s.solve(p.rhs))
+    # a refactor on the same handle (the lookahead events are reused) as well
+    h_l.refactor(p.data)
+    assert np.array_equal(h_l.solve(p.rhs), h_s.solve(p.rhs))
