@@ -237,9 +237,9 @@ __global__ __launch_bounds__(256) void k_trsm(const FrontDesc* __restrict__ fd, 
 // One inner pivot step of one front for one tile of rows (PanelTile):
 //  1. left-looking update from the outer block's columns [J0, j0) of the
 //     diagonal block (C_BB) and of the tile's rows (C_R), FP64 MFMA;
-//  2. LDL^T of the 64 x 64 diagonal block and its inverse L^-1, in the
-//     registers of wave 0 (lane i holds row i; rows are broadcast with
-//     readlane), with the pivot test of src/sparse.rs:528-531;
+//  2. LDL^T of the 64 x 64 diagonal block and its inverse L^-1
+//     (block_ldlt_blk, blocked by 16 columns), with the pivot test of
+//     src/sparse.rs:528-531;
 //  3. the diagonal tile writes D and L^-1 (the arena's pivot block is left as
 //     assembled, because the other tiles of the launch read it); every other
 //     tile applies L[r, B] = A_R L_BB^-T D^-1 to its rows (FP64 MFMA).
@@ -261,13 +261,12 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
   __shared__ double Ad[NB][NB + 1];   // diagonal block [col][row]
   __shared__ double Ar[NB][NB + 1];   // the tile's rows [col][row]
   __shared__ __attribute__((aligned(16))) double Vs[NB][NB + 2];        // L^-1 [col][row]
-  __shared__ __attribute__((aligned(16))) double YY[2][KS][NB + 2];     // Ys, Y0; later L
+  __shared__ __attribute__((aligned(16))) double YY[2][KS][NB + 2];     // Ys, Y0; later scratch
   __shared__ double Xs[KS][NB + 2];   // L[r0 + i, k]
   __shared__ double cbuf[NB], dd[NB];
   double(*Ys)[NB + 2] = YY[0];        // L[j0 + q, k] d_k
   double(*Y0)[NB + 2] = YY[1];        // L[j0 + i, k]
-  static_assert(2 * KS >= NB, "L fits in the update buffers");
-  double(*Ls)[NB + 2] = reinterpret_cast<double(*)[NB + 2]>(&YY[0][0][0]);
+  static_assert(2 * KS * (NB + 2) >= 4 * 256, "the inverse's scratch fits in the update buffers");
   M::acc_t cb[4], cr[4];
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj) {
@@ -320,9 +319,11 @@ __global__ __launch_bounds__(256) void k_panel(const FrontDesc* __restrict__ fd,
       }
     }
   __syncthreads();
-  if (wave == 0) {
+  {
     const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
-    block_ldlt(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, diag_tile ? err : nullptr);
+    // the update buffers are free now: the inverse's scratch
+    block_ldlt_blk(Ad, Vs, &YY[0][0][0], cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0,
+                   diag_tile ? err : nullptr);
   }
   __syncthreads();
   if (diag_tile) {

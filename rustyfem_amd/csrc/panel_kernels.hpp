@@ -8,151 +8,146 @@
 
 namespace rf {
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
-  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-
 // LDL^T of the jb x jb pivot block (jb <= 64) and the inverse of its unit
-// lower factor, by wave 0 alone (lane i owns row i in registers).
-//   in:  A[q][i] = A_BB[i][q], both triangles, zero outside jb x jb
-//   out: dout[i] = D_i; V[q][i] = L^-1[i][q] for q <= i < jb, zero elsewhere;
-//        the pivot test (src/sparse.rs:528-531) against orig = the lane's
-//        original diagonal, the first failing equation eq0 + i wins in *err
-//        (no test when err is null)
-// Step j broadcasts column j of the trailing matrix through LDS (cbuf, one
-// write per lane, uniform-address reads): by symmetry it is row j, the
-// operand of the right-looking update of every row.  The inverse is built
-// one column per lane (lane q solves L x = e_q) from L staged in LDS.  A
-// wave executes its LDS instructions in order, so the exchanges need no
-// barrier; the wave barriers only keep the compiler from reordering them.
-__device__ __forceinline__ void block_ldlt(const double (*A)[NB + 1], double (*Ls)[NB + 2], double (*V)[NB + 2],
-                                  double* cbuf, double* dout, int jb, double orig, int64_t eq0,
-                                  unsigned long long* err) {
-  const int lane = threadIdx.x & 63;
-  double a[NB];
+// lower factor, blocked by 16 columns, by all four waves of the workgroup.
+//   in:  A[q][i] = A_BB[i][q] (column q, row i), both triangles, zero outside
+//        jb x jb
+//   out: A's lower triangle holds L below the diagonal and D on it (the
+//        identity beyond jb); dout[i] = D_i (0 beyond jb); V[q][i] =
+//        L^-1[i][q], zero outside the lower triangle of jb x jb; the pivot
+//        test (src/sparse.rs:528-531) against orig = the original diagonal of
+//        lane i of wave 0, the first failing equation eq0 + i wins in *err (no
+//        test when err is null)
+//   T:   scratch of 4 x 256 doubles
+// Block column b: wave 0 eliminates its 16 columns for every row (lane i
+// owns row i, columns 16b..16b+15 in registers; step j broadcasts column j
+// through LDS, one write per lane and uniform-address reads, as one step of
+// a right-looking elimination); then the waves apply the block's update
+// -L[:, blk] D[blk] L[:, blk]^T to the trailing lower 16 x 16 tiles with FP64
+// MFMA (K = 16).  The inverse: wave 1 inverts the four diagonal 16 x 16
+// blocks (lane 16a + q solves L_aa x = e_q), then the blocks below them
+// follow by distance from the diagonal, V_ab = -V_aa sum_{c=b}^{a-1} L_ac
+// V_cb, one tile per wave on MFMA.  Loops stay short, so the code of the
+// kernels that call this stays small.
+__device__ __forceinline__ void block_ldlt_blk(double (*A)[NB + 1], double (*V)[NB + 2], double* T, double* cbuf,
+                                               double* dout, int jb, double orig, int64_t eq0,
+                                               unsigned long long* err) {
+  using M = Mfma<double>;
+  constexpr int BS = 16, NBK = NB / BS;
+  const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // identity beyond jb, so that every step below is uniform
+  if (tid < NB && tid >= jb) A[tid][tid] = 1.0;
+  for (int idx = tid; idx < NB * NB; idx += 256) V[idx / NB][idx % NB] = 0.0;
+  __syncthreads();
 #pragma unroll
-  for (int q = 0; q < NB; ++q) a[q] = A[q][lane];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < jb) {
-      cbuf[lane] = a[j];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // every read of the step is issued before the first product
-      double w[NB];
-#pragma unroll
-      for (int q = j; q < NB; ++q) w[q] = cbuf[q];
-      __builtin_amdgcn_sched_barrier(0);
-      const double l = a[j] / w[j];
-#pragma unroll
-      for (int q = j + 1; q < NB; ++q) a[q] = fma(-l, w[q], a[q]);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (lane > j) a[j] = l;
-    }
-  }
-  // lane j holds D_j in a[j]
-  double dl = 0.0;
-#pragma unroll
-  for (int q = 0; q < NB; ++q)
-    if (q == lane) dl = a[q];
-  if (err != nullptr && lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
-    atomicMin(err, static_cast<unsigned long long>(eq0 + lane));
-  dout[lane] = lane < jb ? dl : 0.0;
-#pragma unroll
-  for (int p = 0; p < NB; ++p) Ls[lane][p] = (p < lane && lane < jb) ? a[p] : 0.0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  double x[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = i == lane ? 1.0 : 0.0;
-#pragma unroll
-  for (int i = 1; i < NB; ++i) {
-    if (i < jb) {
-      double w[NB];
-#pragma unroll
-      for (int p = 0; p < i; ++p) w[p] = Ls[i][p];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int p = 0; p < i; ++p) x[i] = fma(-w[p], x[p], x[i]);
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < NB; ++i) V[lane][i] = (lane < jb && i < jb && i >= lane) ? x[i] : 0.0;
-}
-
-// block_ldlt with the inverse pipelined behind the factorization, called by
-// every wave of the workgroup: in iteration j wave 0 runs elimination step j
-// and publishes column j of L to Ls, while wave 1 computes row j of L^-1
-// (row j of L is final after step j - 1); one workgroup barrier per
-// iteration.  The two waves keep their row / column in the same register
-// array.  The arithmetic, and so every result, is that of block_ldlt.
-__device__ __forceinline__ void block_ldlt_pipe(const double (*A)[NB + 1], double (*Ls)[NB + 2], double (*V)[NB + 2],
-                                                double* cbuf, double* dout, int jb, double orig, int64_t eq0,
-                                                unsigned long long* err) {
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  double r[NB];  // wave 0: row `lane` of the block; wave 1: column `lane` of L^-1
-  if (wave == 0) {
-#pragma unroll
-    for (int q = 0; q < NB; ++q) r[q] = A[q][lane];
-  } else {
-#pragma unroll
-    for (int q = 0; q < NB; ++q) r[q] = q == lane ? 1.0 : 0.0;
-  }
-  __syncthreads();  // A has been read: Ls shares its bytes
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
+  for (int b = 0; b < NBK; ++b) {
+    const int c0 = BS * b;
     if (wave == 0) {
-      double lj = 0.0;
-      if (j < jb) {
-        cbuf[lane] = r[j];
+      double r[BS];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) r[q] = A[c0 + q][lane];
+#pragma unroll
+      for (int jj = 0; jj < BS; ++jj) {
+        cbuf[lane] = r[jj];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double w[NB];
+        double w[BS];
 #pragma unroll
-        for (int q = j; q < NB; ++q) w[q] = cbuf[q];
+        for (int q = jj; q < BS; ++q) w[q] = cbuf[c0 + q];
         __builtin_amdgcn_sched_barrier(0);
-        const double l = r[j] / w[j];
+        const double l = r[jj] / w[jj];
 #pragma unroll
-        for (int q = j + 1; q < NB; ++q) r[q] = fma(-l, w[q], r[q]);
+        for (int q = jj + 1; q < BS; ++q) r[q] = fma(-l, w[q], r[q]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane > j) r[j] = l;
-        lj = l;
+        if (lane > c0 + jj) r[jj] = l;
       }
-      Ls[lane][j] = (j < jb && lane > j && lane < jb) ? lj : 0.0;
-    } else if (wave == 1) {
-      if (j >= 1 && j < jb) {
-        double w[NB];
+      if (lane >= c0) {
 #pragma unroll
-        for (int p = 0; p < j; ++p) w[p] = Ls[j][p];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int p = 0; p < j; ++p) r[j] = fma(-w[p], r[p], r[j]);
+        for (int q = 0; q < BS; ++q) A[c0 + q][lane] = r[q];
       }
     }
     __syncthreads();
-  }
-  if (wave == 0) {
-    double dl = 0.0;
+    // trailing lower tiles (ti, tc), b < tc <= ti < NBK, one per wave
+    const int nt = NBK - 1 - b;
+    for (int tt = wave; tt < nt * (nt + 1) / 2; tt += 4) {
+      int ti = b + 1, rem = tt;
+      while (rem >= ti - b) {
+        rem -= ti - b;
+        ++ti;
+      }
+      const int tc = b + 1 + rem;
+      // output (row 16 ti + g + 4 r, column 16 tc + l16) in register r
+      M::acc_t acc;
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (q == lane) dl = r[q];
+      for (int r = 0; r < 4; ++r) acc[r] = A[BS * tc + l16][BS * ti + g + 4 * r];
+#pragma unroll
+      for (int s4 = 0; s4 < BS / 4; ++s4) {
+        const int k = c0 + 4 * s4 + g;
+        acc = M::op(-A[k][BS * ti + l16] * A[k][k], A[k][BS * tc + l16], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) A[BS * tc + l16][BS * ti + g + 4 * r] = acc[r];
+    }
+    __syncthreads();
+  }
+  // D and the pivot test
+  if (wave == 0) {
+    const double dl = A[lane][lane];
     if (err != nullptr && lane < jb && (!(orig > 0.0) || !(dl > 1e-12 * orig)))
       atomicMin(err, static_cast<unsigned long long>(eq0 + lane));
     dout[lane] = lane < jb ? dl : 0.0;
   } else if (wave == 1) {
+    // column q of L_aa^-1 (unit lower), a = g
+    const int r0 = BS * g;
+    double x[BS];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) V[lane][i] = (lane < jb && i < jb && i >= lane) ? r[i] : 0.0;
+    for (int i = 0; i < BS; ++i) x[i] = i == l16 ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 1; i < BS; ++i)
+#pragma unroll
+      for (int p = 0; p < i; ++p) x[i] = fma(-A[r0 + p][r0 + i], x[p], x[i]);
+#pragma unroll
+    for (int i = 0; i < BS; ++i) V[r0 + l16][r0 + i] = x[i];
+  }
+  __syncthreads();
+  for (int dist = 1; dist < NBK; ++dist) {
+    const int a = dist + wave;
+    if (a < NBK) {
+      const int b = a - dist;
+      // T = sum_{c=b}^{a-1} L_ac V_cb; register r holds T[g + 4 r][l16]
+      M::acc_t t = M::acc_t{0, 0, 0, 0};
+      for (int c = b; c < a; ++c)
+#pragma unroll
+        for (int s4 = 0; s4 < BS / 4; ++s4) {
+          const int k = BS * c + 4 * s4 + g;
+          t = M::op(A[k][BS * a + l16], V[BS * b + l16][k], t);
+        }
+      double* Tw = T + wave * BS * BS;  // Tw[j * 16 + i] = T[i][j]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Tw[l16 * BS + g + 4 * r] = t[r];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // V_ab = -V_aa T
+      M::acc_t o = M::acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int s4 = 0; s4 < BS / 4; ++s4) {
+        const int k = 4 * s4 + g;
+        o = M::op(V[BS * a + k][BS * a + l16], Tw[l16 * BS + k], o);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) V[BS * b + l16][BS * a + g + 4 * r] = -o[r];
+    }
+    __syncthreads();
+  }
+  // zero outside jb x jb (the identity padding and the signs of its zeros)
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int q = idx / NB, i = idx % NB;
+    if (q >= jb || i >= jb || i < q) V[q][i] = 0.0;
   }
 }
 
@@ -172,8 +167,8 @@ struct PanelPass {
 // t), the order of the fused panel kernel, so both give identical factors.
 //
 // k_pdiag (one workgroup per front): C_BB = Y D Y^T with Y = L[B, J0:j0],
-// A_BB = F_BB - C_BB, then block_ldlt in wave 0; writes D and V = L_BB^-1
-// (the arena's pivot block stays as assembled).
+// A_BB = F_BB - C_BB, then block_ldlt_blk; writes D and V = L_BB^-1 (the
+// arena's pivot block stays as assembled).
 template <typename T>
 __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
                                                const T* __restrict__ arena, double* __restrict__ dpiv,
@@ -183,16 +178,15 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
   constexpr int PK = PanelPass<T>::PK;
   constexpr int NS = PK / 4;
   constexpr int YBYTES = PK * (NB + 2) * static_cast<int>(sizeof(T)) + PK * 8;
-  constexpr int ABYTES = NB * (NB + 1) * 8;  // Ad, then L (pitch NB + 2) in the same bytes
-  constexpr int LBYTES = NB * (NB + 2) * 8;
+  constexpr int ABYTES = NB * (NB + 1) * 8;  // Ad, factored in place
+  constexpr int TOFF = (ABYTES + 15) / 16 * 16;  // the inverse's scratch after Ad
+  constexpr int LBYTES = TOFF + 4 * 256 * 8;
   __shared__ __attribute__((aligned(16))) unsigned char smem[YBYTES > LBYTES ? YBYTES : LBYTES];
   __shared__ __attribute__((aligned(16))) double Vs[NB][NB + 2];  // Vs[q][i] = L^-1[i][q]
   __shared__ double cbuf[NB], dd[NB];
-  static_assert(ABYTES <= LBYTES, "Ad and L share storage");
   T(*Yk)[NB + 2] = reinterpret_cast<T(*)[NB + 2]>(smem);                            // Yk[k][i] = L[j0 + i, p0 + k]
   double* Ds = reinterpret_cast<double*>(smem + PK * (NB + 2) * sizeof(T));          // D[p0 + k]
   double(*Ad)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);                  // after the update
-  double(*Ls)[NB + 2] = reinterpret_cast<double(*)[NB + 2]>(smem);                  // after Ad is in registers
   const PanelTile t = tiles[blockIdx.x];
   const FrontDesc d = fd[t.front];
   const int jb = t.jb, tid = threadIdx.x, lane = tid & 63;
@@ -270,7 +264,8 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
       Ad[q][i] = (i < jb && q < jb) ? static_cast<double>(fb[bj][r]) - cb[bj][r] : 0.0;
     }
   __syncthreads();
-  block_ldlt_pipe(Ad, Ls, Vs, cbuf, dd, jb, orig, static_cast<int64_t>(d.start) + t.j0, err);
+  block_ldlt_blk(Ad, Vs, reinterpret_cast<double*>(smem + TOFF), cbuf, dd, jb, orig,
+                 static_cast<int64_t>(d.start) + t.j0, err);
   __syncthreads();
   double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
