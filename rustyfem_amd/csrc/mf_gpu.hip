@@ -1389,6 +1389,9 @@ class Impl final : public GpuFactor {
       // RUSTYFEM_AUTOTUNE: unset or 1 = time the three GEMM kernels per
       // launch, 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere,
       // "ilv" = k_gemm_ilv everywhere
+      // RUSTYFEM_TUNE_MARGIN: a variant must be this fraction faster to be picked
+      const char* tm = std::getenv("RUSTYFEM_TUNE_MARGIN");
+      if (tm && *tm) tune_margin_ = 1.0f - static_cast<float>(std::atof(tm));
       const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
       const int8_t force = a && std::strcmp(a, "agpr") == 0 ? 1 : a && std::strcmp(a, "ilv") == 0 ? 2 : 0;
       autotune_ = !(a && *a == '0') && force == 0;
@@ -1681,7 +1684,8 @@ class Impl final : public GpuFactor {
   // Autotuning of the GEMM kernel: every GEMM launch of the plan runs the
   // three variants (one warm-up, then three timed launches each, the minimum
   // kept); a variant replaces the current choice (k_gemm first) only where it
-  // is at least 3% faster.  All three give identical results.  The launches
+  // is at least 1% faster (RUSTYFEM_TUNE_MARGIN).  All three give identical
+  // results.  The launches
   // work on the arena before the first factorization assembles anything in
   // it; the factorization clears the arena and every level assembles its
   // fronts before its own GEMMs, so
@@ -1720,7 +1724,7 @@ class Impl final : public GpuFactor {
       }
       int pick = 0;
       for (int v = 1; v < NV; ++v)
-        if (best[v] < 0.97f * best[pick]) pick = v;
+        if (best[v] < tune_margin_ * best[pick]) pick = v;
       gemm_pick_[ph[i]->r.begin] = static_cast<int8_t>(pick);
       ++count[pick];
     }
@@ -2004,6 +2008,7 @@ class Impl final : public GpuFactor {
   hipStream_t s2_ = nullptr;
   hipEvent_t ev_look_ = nullptr, ev_rest_ = nullptr;
   bool lookahead_ = true, rest_pending_ = false;
+  float tune_margin_ = 0.99f;  // a GEMM variant replaces the current one when its time is below this factor
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   int64_t n_ = 0, nlower_ = 0, nfull_ = 0;
   size_t bytes_ = 0;
