@@ -121,6 +121,7 @@ struct LevelPlan {
   std::vector<SbFwdStep> sfwd;     // super-block forward steps
   std::vector<SbBwdStep> sbwd;     // super-block backward steps (last super-block first)
   Range fronts_r;  // range in the level-ordered front list (solve kernels)
+  Range zero;      // ZeroTile: the lower triangles of the level's fronts
 };
 
 struct Plan {
@@ -131,7 +132,7 @@ struct Plan {
   std::vector<int32_t> level_fronts;  // fronts grouped by level (solve kernels)
   std::vector<int32_t> cmap;          // child row -> parent local index
   std::vector<int64_t> amap;          // Lower entry -> arena offset
-  std::vector<ZeroTile> zero;          // lower triangles of all fronts, ZC columns per tile
+  std::vector<ZeroTile> zero;          // lower triangles of all fronts, ZC columns per tile, level by level
   std::vector<ScatterTile> scatter;
   std::vector<EaTile> ea;
   std::vector<GemmTile> gemm;

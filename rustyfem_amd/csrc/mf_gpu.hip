@@ -1487,9 +1487,29 @@ class Impl final : public GpuFactor {
     if (autotune_ && !tuned_) tune_gemm();
     nev_ = 0;
     tm_.gemm_flops = 0;
-    launch(k_zero<T>, static_cast<int64_t>(P_.zero.size()), 256, d_fd_, d_zero_, d_arena_);
-    for (const LevelPlan& lp : P_.levels) {
+    // each level's fronts are cleared before the level: level 0 here, level
+    // L + 1 on the second stream while level L works (the level's start
+    // waits for that stream)
+    if (lookahead_) {
+      // the second stream starts after everything issued before (a solve
+      // still reading the arena, the autotuning)
+      HIPCHK(hipEventRecord(ev_look_, s_));
+      HIPCHK(hipStreamWaitEvent(s2_, ev_look_, 0));
+    }
+    if (!P_.levels.empty()) {
+      const Range& z = P_.levels[0].zero;
+      launch(k_zero<T>, lookahead_ ? z.count : static_cast<int64_t>(P_.zero.size()), 256, d_fd_, d_zero_ + z.begin,
+             d_arena_);
+    }
+    for (size_t L = 0; L < P_.levels.size(); ++L) {
+      const LevelPlan& lp = P_.levels[L];
       join_rest();
+      if (lookahead_ && L + 1 < P_.levels.size() && P_.levels[L + 1].zero.count > 0) {
+        const Range& z = P_.levels[L + 1].zero;
+        launch_on(s2_, k_zero<T>, z.count, 256, d_fd_, d_zero_ + z.begin, d_arena_);
+        HIPCHK(hipEventRecord(ev_rest_, s2_));
+        rest_pending_ = true;
+      }
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);

@@ -51,8 +51,6 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     P.flops += k * k * k / 3.0 + uu * k * k + uu * uu * k;
   }
   P.arena_elems = off;
-  for (int64_t f = 0; f < P.nf; ++f)
-    for (int32_t c0 = 0; c0 < P.fd[f].m; c0 += ZC) P.zero.push_back({static_cast<int32_t>(f), c0});
   P.tvec_elems = toff;
   // child row -> parent local index
   P.cmap.resize(ncmap);
@@ -110,6 +108,11 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
     int64_t b = static_cast<int64_t>(P.level_fronts.size());
     P.level_fronts.insert(P.level_fronts.end(), lp.fronts.begin(), lp.fronts.end());
     lp.fronts_r = take(b, static_cast<int64_t>(P.level_fronts.size()));
+    // clearing tiles of the level's fronts
+    b = static_cast<int64_t>(P.zero.size());
+    for (int32_t f : lp.fronts)
+      for (int32_t c0 = 0; c0 < P.fd[f].m; c0 += ZC) P.zero.push_back({f, c0});
+    lp.zero = take(b, static_cast<int64_t>(P.zero.size()));
     // scatter tiles over the Lower entries of the level's pivot columns
     b = static_cast<int64_t>(P.scatter.size());
     for (int32_t f : lp.fronts) {

@@ -171,6 +171,19 @@ std::string check_plan(const Plan& P, int plant) {
   }
   for (size_t L = 0; L < P.levels.size(); ++L) {
     const LevelPlan& lp = P.levels[L];
+    // a level's clearing launch runs beside the previous level's launches:
+    // it may touch only the level's own fronts, with the tile count of each
+    {
+      std::vector<int32_t> cols(P.nf, 0);
+      for (int64_t t = lp.zero.begin; t < lp.zero.begin + lp.zero.count; ++t) {
+        const ZeroTile& z = P.zero[t];
+        if (z.front < 0 || z.front >= P.nf || P.level_of[z.front] != static_cast<int32_t>(L) || z.c0 % ZC != 0)
+          return "level " + std::to_string(L) + " clearing tile " + std::to_string(t) + " outside the level";
+        cols[z.front] += 1;
+      }
+      for (int32_t f : lp.fronts)
+        if (cols[f] != (P.fd[f].m + ZC - 1) / ZC) return "level " + std::to_string(L) + " front not cleared exactly once";
+    }
     // extend-add: each launch writes parent entries (map[x], map[y]); they must be distinct
     for (int s = 0; s < 2; ++s) {
       std::vector<std::pair<int64_t, int32_t>> w;  // (arena offset, tile)
