@@ -1284,11 +1284,22 @@ __global__ __launch_bounds__(256) void k_seg_sumsq(const double* __restrict__ v,
     }
 }
 
-__global__ void k_sumsq_final(const double* __restrict__ part, int nblk, int nrhs, double* __restrict__ out) {
-  for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[j * nblk + b];
-    out[j] = s;
+// one workgroup of 64 per right-hand side j (blockIdx.x): lane t adds the
+// partials t, t + 64, ... in order, then lane 0 adds the 64 lane sums in
+// order (a fixed order: the same sums on every call)
+__global__ __launch_bounds__(64) void k_sumsq_final(const double* __restrict__ part, int nblk, int nrhs,
+                                                    double* __restrict__ out) {
+  __shared__ double ls[64];
+  const int j = blockIdx.x;
+  if (j >= nrhs) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) s += part[static_cast<int64_t>(j) * nblk + b];
+  ls[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < 64; ++i) t += ls[i];
+    out[j] = t;
   }
 }
 
@@ -1837,7 +1848,7 @@ class Impl final : public GpuFactor {
 
   void sumsq(const double* v, int nrhs, double* host_out) {
     launch(k_sumsq_partial, kRedBlocks, 256, v, n_, nrhs, d_red_);
-    launch(k_sumsq_final, 1, 64, d_red_, kRedBlocks, nrhs, d_nrm_);
+    launch(k_sumsq_final, nrhs, 64, d_red_, kRedBlocks, nrhs, d_nrm_);
     HIPCHK(hipMemcpyAsync(host_out, d_nrm_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIPCHK(hipStreamSynchronize(s_));
   }
