@@ -1524,20 +1524,28 @@ class Impl final : public GpuFactor {
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
       for (int sl = 0; sl < 2; ++sl)
         launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
-      for (const Phase& ph : lp.phases) {
+      for (size_t pi = 0; pi < lp.phases.size(); ++pi) {
+        const Phase& ph = lp.phases[pi];
         // only the pivot-block phases may run beside the rest of an update
         if (ph.kind != PhaseKind::PanelDiag && ph.kind != PhaseKind::PanelRows && ph.kind != PhaseKind::Panel)
           join_rest();
+        const bool rest_next = lookahead_ && pi + 1 < lp.phases.size() &&
+                               lp.phases[pi + 1].kind == PhaseKind::GemmRest && lp.phases[pi + 1].r.count > 0;
         switch (ph.kind) {
           case PhaseKind::Gemm64:
           case PhaseKind::Gemm128:
+            // the rest of this update reads the same finished panel and
+            // writes other columns: it may start beside this launch
+            if (rest_next) HIPCHK(hipEventRecord(ev_look_, s_));
             gemm(ph, s_);
             break;
           case PhaseKind::GemmRest:
             if (lookahead_ && ph.r.count > 0) {
-              // on the second stream, after everything issued so far on the
-              // main one; the next panel phases go on beside it
-              HIPCHK(hipEventRecord(ev_look_, s_));
+              // on the second stream, after everything issued on the main
+              // one before the lookahead part; the next panel phases go on
+              // beside it
+              if (pi == 0 || lp.phases[pi - 1].kind != PhaseKind::Gemm128 || lp.phases[pi - 1].r.count == 0)
+                HIPCHK(hipEventRecord(ev_look_, s_));
               HIPCHK(hipStreamWaitEvent(s2_, ev_look_, 0));
               gemm(ph, s2_);
               HIPCHK(hipEventRecord(ev_rest_, s2_));

@@ -1,8 +1,9 @@
 // Static hazard check of a factorization plan: inside one launch, no tile may
 // write an arena element that another tile of the same launch reads or
 // writes; and the rest of an outer block's update (a GemmRest launch, which
-// runs on a second stream) may not write what the panel launches running
-// beside it read or write, nor read what they write.  Footprints are
+// runs on a second stream) may not write what the launches running beside it
+// (the lookahead part of the same update, the next panel launches) read or
+// write, nor read what they write.  Footprints are
 // rectangles (front, rows, columns) of the column-major fronts; the
 // extend-add footprints are sets of parent entries reached through the
 // child->parent row map.
@@ -265,6 +266,11 @@ std::string check_plan(const Plan& P, int plant) {
       if (!e.empty()) return "level " + std::to_string(L) + " " + e;
       if (ph.kind != PhaseKind::GemmRest) continue;
       const std::vector<Rect> rest = rects_of(ph);
+      // beside the lookahead part of the same update, which precedes it
+      if (i > 0 && lp.phases[i - 1].kind == PhaseKind::Gemm128) {
+        const std::string c = check_cross(rest, rects_of(lp.phases[i - 1]), "lookahead");
+        if (!c.empty()) return "level " + std::to_string(L) + " " + c;
+      }
       for (size_t j = i + 1; j < lp.phases.size() && is_panel(lp.phases[j].kind); ++j) {
         const std::string c = check_cross(rest, rects_of(lp.phases[j]), "lookahead");
         if (!c.empty()) return "level " + std::to_string(L) + " " + c;
