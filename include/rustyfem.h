@@ -104,8 +104,8 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
  * [15] how many of them it moved to the AGPR-accumulator kernel,
  * [16..19] the worst relative residual after 0, 1, 2 and 3 refinement
  * corrections of the last solve (-1 where not reached), [20] how many GEMM
- * launches the autotuning moved to the interleaved-load kernel.  At most 21
- * entries are written. */
+ * launches the autotuning moved to the kernels with double-width K tiles.
+ * At most 21 entries are written. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */

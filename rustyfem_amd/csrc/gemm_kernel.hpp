@@ -142,7 +142,7 @@ __device__ __forceinline__ void gemm_tile(const FrontDesc* __restrict__ fd, cons
 // Every accumulator receives the same MFMA sequence as in gemm_tile (k in
 // increasing order, four k per 16-byte LDS read in FP32, two in FP64): both
 // give identical results.
-template <typename T>
+template <typename T, bool PATTERN>
 __device__ __forceinline__ void gemm_tile_ilv(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
                                               T* __restrict__ arena, const double* __restrict__ dpiv) {
   using M = Mfma<T>;
@@ -235,11 +235,13 @@ __device__ __forceinline__ void gemm_tile_ilv(const FrontDesc* __restrict__ fd, 
     products();
     // one load (and the two scalar instructions of its address) after each
     // of the first 8 NQ products
+    if constexpr (PATTERN) {
 #pragma unroll
-    for (int i = 0; i < 8 * NQ; ++i) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x004, 2, 0);
-      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      for (int i = 0; i < 8 * NQ; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x004, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
     }
   }
   __syncthreads();
@@ -287,11 +289,20 @@ __global__ __launch_bounds__(256) void k_gemm_agpr(const FrontDesc* __restrict__
   gemm_tile<T>(fd, tiles, arena, dpiv);
 }
 
-// gemm_tile_ilv, two waves per SIMD
+// gemm_tile_ilv, two waves per SIMD, with the load interleaving pattern
+// (k_gemm_ilv) and without it, the loads where the compiler puts them
+// (k_gemm_k64)
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_ilv(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
                                                   T* __restrict__ arena, const double* __restrict__ dpiv) {
-  gemm_tile_ilv<T>(fd, tiles, arena, dpiv);
+  gemm_tile_ilv<T, true>(fd, tiles, arena, dpiv);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_k64(
+    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles, T* __restrict__ arena,
+    const double* __restrict__ dpiv) {
+  gemm_tile_ilv<T, false>(fd, tiles, arena, dpiv);
 }
 
 }  // namespace rf

@@ -1397,14 +1397,17 @@ class Impl final : public GpuFactor {
       chain_solve_ = c && *c && *c != '0';
       const char* rb = std::getenv("RUSTYFEM_SOLVE_RHS_BLOCK");
       if (rb && *rb) rhs_block_ = std::atoi(rb);
-      // RUSTYFEM_AUTOTUNE: unset or 1 = time the three GEMM kernels per
-      // launch, 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere,
-      // "ilv" = k_gemm_ilv everywhere
       // RUSTYFEM_TUNE_MARGIN: a variant must be this fraction faster to be picked
       const char* tm = std::getenv("RUSTYFEM_TUNE_MARGIN");
       if (tm && *tm) tune_margin_ = 1.0f - static_cast<float>(std::atof(tm));
+      // RUSTYFEM_AUTOTUNE: unset or 1 = time the four GEMM kernels per
+      // launch, 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere,
+      // "ilv" = k_gemm_ilv everywhere, "k64" = k_gemm_k64 everywhere
       const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
-      const int8_t force = a && std::strcmp(a, "agpr") == 0 ? 1 : a && std::strcmp(a, "ilv") == 0 ? 2 : 0;
+      const int8_t force = a && std::strcmp(a, "agpr") == 0  ? 1
+                           : a && std::strcmp(a, "ilv") == 0 ? 2
+                           : a && std::strcmp(a, "k64") == 0 ? 3
+                                                             : 0;
       autotune_ = !(a && *a == '0') && force == 0;
       gemm_pick_.assign(P_.gemm.size(), force);
     }
@@ -1710,20 +1713,22 @@ class Impl final : public GpuFactor {
     }
   }
 
-  // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv
+  // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv, 3 = k_gemm_k64
   void launch_gemm(int variant, const Range& r, hipStream_t st) {
     if (variant == 1)
       launch_on(st, k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     else if (variant == 2)
       launch_on(st, k_gemm_ilv<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else if (variant == 3)
+      launch_on(st, k_gemm_k64<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     else
       launch_on(st, k_gemm<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
   }
 
   // Autotuning of the GEMM kernel: every GEMM launch of the plan runs the
-  // three variants (one warm-up, then three timed launches each, the minimum
+  // four variants (one warm-up, then three timed launches each, the minimum
   // kept); a variant replaces the current choice (k_gemm first) only where it
-  // is at least 1% faster (RUSTYFEM_TUNE_MARGIN).  All three give identical
+  // is at least 1% faster (RUSTYFEM_TUNE_MARGIN).  All four give identical
   // results.  The launches
   // work on the arena before the first factorization assembles anything in
   // it; the factorization clears the arena and every level assembles its
@@ -1734,7 +1739,7 @@ class Impl final : public GpuFactor {
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
         if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0) ph.push_back(&p);
-    constexpr int NV = 3;
+    constexpr int NV = 4;
     std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(e0_, s_));
@@ -1750,7 +1755,7 @@ class Impl final : public GpuFactor {
     }
     HIPCHK(hipEventRecord(e1_, s_));
     HIPCHK(hipStreamSynchronize(s_));
-    int64_t count[NV] = {0, 0, 0};
+    int64_t count[NV] = {0, 0, 0, 0};
     for (size_t i = 0; i < ph.size(); ++i) {
       float best[NV];
       for (int v = 0; v < NV; ++v) {
@@ -1773,7 +1778,7 @@ class Impl final : public GpuFactor {
     tm_.tune_ms = ms;
     tm_.tuned_launches = static_cast<double>(ph.size());
     tm_.agpr_launches = static_cast<double>(count[1]);
-    tm_.ilv_launches = static_cast<double>(count[2]);
+    tm_.ilv_launches = static_cast<double>(count[2] + count[3]);
     tuned_ = true;
   }
 
