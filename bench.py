@@ -509,9 +509,10 @@ def main():
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
             "frac": (achieved / PEAK["fp32_mfma_tflops"]) if achieved else None, "traffic": measured_traffic(),
-            "kernel": "k_gemm<float> / k_gemm_agpr<float> / k_gemm_ilv<float> (Schur update, "
+            "kernel": "k_gemm<float> / k_gemm_agpr<float> / k_gemm_ilv<float> / k_gemm_k64<float> (Schur update, "
                       "v_mfma_f32_16x16x4f32; register layout and K tile autotuned per launch)",
-            "autotuned_agpr_launches": st.get("agpr_launches"), "autotuned_ilv_launches": st.get("ilv_launches"),
+            "autotuned_agpr_launches": st.get("agpr_launches"),
+            "autotuned_double_width_k_launches": st.get("ilv_launches"),
             "autotune_ms": st.get("tune_ms"),
             "launches": gemm_n, "avg_launch_ms": gemm_ms / gemm_n if gemm_n else None,
             "algorithmic_flops_per_launch": gemm_flops / gemm_n if gemm_n else None}
