@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -93,9 +94,18 @@ struct Vec {
 // val[((w / GR) nnzb + q) GR + w % GR] (granules of GR entries interleaved
 // across blocks: the G lanes of a row, which take consecutive blocks, read
 // consecutive granules, one vector load each).
-template <typename T, int BR, int BC, int G, int MODE, int GR>
+// a stored block value as T: bfloat16 (kept as its 16 bits) widens exactly to FP32
+template <typename T, typename VS>
+__device__ __forceinline__ T widen(VS v) {
+  if constexpr (std::is_same_v<VS, uint16_t>)
+    return __builtin_bit_cast(float, static_cast<uint32_t>(v) << 16);
+  else
+    return static_cast<T>(v);
+}
+
+template <typename T, int BR, int BC, int G, int MODE, int GR, typename VS = T>
 __global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, int64_t nnzb, const int64_t* __restrict__ ptr,
-                                             const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                             const int32_t* __restrict__ idx, const VS* __restrict__ val,
                                              const T* __restrict__ x, T* y, const T* b, const int32_t* done) {
   if (done && *done) return;
   const int64_t g = (static_cast<int64_t>(blockIdx.x) * RB + threadIdx.x) / G;
@@ -113,14 +123,14 @@ __global__ __launch_bounds__(RB) void k_spmv(int64_t nbr, int64_t nnzb, const in
 #pragma unroll
       for (int i = 0; i < BR; ++i)
 #pragma unroll
-        for (int j = 0; j < BC; ++j) s[i] += val[q * (BR * BC) + i * BC + j] * xv[j];
+        for (int j = 0; j < BC; ++j) s[i] += widen<T>(val[q * (BR * BC) + i * BC + j]) * xv[j];
     } else {
-      using VT = typename Vec<T, GR>::type;
+      using VT = typename Vec<VS, GR>::type;
 #pragma unroll
       for (int gi = 0; gi < BR * BC / GR; ++gi) {
         const VT v = *reinterpret_cast<const VT*>(val + (static_cast<int64_t>(gi) * nnzb + q) * GR);
 #pragma unroll
-        for (int e = 0; e < GR; ++e) s[(gi * GR + e) / BC] += v[e] * xv[(gi * GR + e) % BC];
+        for (int e = 0; e < GR; ++e) s[(gi * GR + e) / BC] += widen<T>(v[e]) * xv[(gi * GR + e) % BC];
       }
     }
   }
@@ -469,9 +479,11 @@ struct DevMat {
   bool soa = false;          // granule-interleaved block values (see k_spmv)
   int64_t nnzb = 0;
   DBuf ptr, idx, v64, v32;
+  DBuf v16;  // bfloat16 values for the FP32 V-cycle (coarse levels, when asked for)
 
   size_t bytes_per_product(bool fp32) const {
-    return static_cast<size_t>(nnzb) * (bs * bsc * (fp32 ? 4 : 8) + 4) + static_cast<size_t>(nbr + 1) * 8 +
+    const int vb = fp32 ? (v16.p ? 2 : 4) : 8;
+    return static_cast<size_t>(nnzb) * (bs * bsc * vb + 4) + static_cast<size_t>(nbr + 1) * 8 +
            static_cast<size_t>(nbr * bs + nbc * bsc) * (fp32 ? 4 : 8);
   }
 };
@@ -494,7 +506,23 @@ int pick_group(double per_row) {
 
 // scalar CSR -> bs x bsc blocks (every entry placed, missing ones zero); the
 // row and column counts must be multiples of the block sizes
-DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32, bool soa = false) {
+// The FP32 V-cycle keeps the values of its coarse operators and transfers in
+// bfloat16 (half the bytes of those products; the arithmetic stays FP32, the
+// finest level FP32, the Krylov loop FP64); RUSTYFEM_VCYCLE_BF16=0 keeps FP32
+bool vcycle_bf16() {
+  const char* e = std::getenv("RUSTYFEM_VCYCLE_BF16");
+  return !(e && *e == '0');
+}
+
+// float -> bfloat16, round to nearest even (finite values)
+uint16_t to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32, bool soa = false, bool keep16 = false) {
   if (a.nrows % bs || a.ncols % bsc) throw std::runtime_error("internal: block sizes do not divide the matrix");
   DevMat M;
   M.bs = bs;
@@ -548,6 +576,12 @@ DevMat make_mat(const Csr& a, int bs, int bsc, bool keep64, bool keep32, bool so
     std::vector<float> f(t.begin(), t.end());
     M.v32 = upload(f);
   }
+  if (keep16) {
+    const std::vector<double> t = M.soa ? interleave(granule(bb, 2)) : bval;
+    std::vector<uint16_t> h(t.size());
+    for (size_t i = 0; i < t.size(); ++i) h[i] = to_bf16(static_cast<float>(t[i]));
+    M.v16 = upload(h);
+  }
   return M;
 }
 
@@ -563,6 +597,22 @@ void launch_spmv_g(const DevMat& M, int mode, const T* x, T* y, const T* b, cons
   const unsigned grid = grid_for(M.nbr * G);
   const auto* P = M.ptr.as<int64_t>();
   const auto* I = M.idx.as<int32_t>();
+  if constexpr (std::is_same_v<T, float>) {
+    if (M.v16.p) {
+      const auto* H = M.v16.as<uint16_t>();
+      constexpr int S = granule(BR * BC, 2);
+      if (M.soa && BR * BC > 1) {
+        if (mode == 0) k_spmv<T, BR, BC, G, 0, S, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+        else if (mode == 1) k_spmv<T, BR, BC, G, 1, S, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+        else k_spmv<T, BR, BC, G, 2, S, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+        return;
+      }
+      if (mode == 0) k_spmv<T, BR, BC, G, 0, 0, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+      else if (mode == 1) k_spmv<T, BR, BC, G, 1, 0, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+      else k_spmv<T, BR, BC, G, 2, 0, uint16_t><<<grid, RB, 0, s>>>(M.nbr, M.nnzb, P, I, H, x, y, b, done);
+      return;
+    }
+  }
   const T* V = vals<T>(M);
   if (M.soa && BR * BC > 1) {
     constexpr int S = granule(BR * BC, static_cast<int>(sizeof(T)));
@@ -761,8 +811,8 @@ void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& c
     // transfer operators in blocks of the two levels' DOFs per node/aggregate
     const int bf = blk_[l], bcs = blk_[l + 1];
     const bool ok = spmv_shape_ok(bf, bcs) && spmv_shape_ok(bcs, bf);
-    V.P = make_mat(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32, true);
-    V.R = make_mat(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32, true);
+    V.P = make_mat(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32, true, f32 && vcycle_bf16());
+    V.R = make_mat(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32, true, f32 && vcycle_bf16());
     std::vector<T> di(V.n);
     for (int64_t i = 0; i < V.n; ++i) di[i] = static_cast<T>(1.0 / L.diag[i]);
     V.dinv = upload(di);
@@ -831,7 +881,7 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
     for (size_t l = 1; l < H.levels.size(); ++l) blk_.push_back(uniform_block(H.levels[l].bptr));
     blk_.push_back(uniform_block(H.coarse_bptr));
     for (size_t l = 1; l < H.levels.size(); ++l)
-      coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true));
+      coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16()));
     if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
     else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
   }
