@@ -78,7 +78,12 @@ def test_fcg_reproduces_pcg(rf, block12):
     assert np.max(np.abs(x1 - x2)) / np.max(np.abs(x1)) < 1e-9
 
 
-def test_fp32_vcycle_fcg_meets_fp64_target(rf, ora, block12):
+@pytest.mark.parametrize("bf16", ["1", "0"])
+def test_fp32_vcycle_fcg_meets_fp64_target(rf, ora, block12, monkeypatch, bf16):
+    """FCG with the FP32 V-cycle, its coarse operators and transfers stored in
+    bfloat16 (the default) or in FP32 (RUSTYFEM_VCYCLE_BF16=0), reaches the
+    FP64 target and the direct solution."""
+    monkeypatch.setenv("RUSTYFEM_VCYCLE_BF16", bf16)
     p = block12
     b = p.rhs[:, 0].copy()
     s, _ = setup(rf, p, krylov="fcg", vcycle="fp32")
