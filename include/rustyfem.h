@@ -104,8 +104,14 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
  * [15] how many of them it moved to the AGPR-accumulator kernel,
  * [16..19] the worst relative residual after 0, 1, 2 and 3 refinement
  * corrections of the last solve (-1 where not reached), [20] how many GEMM
- * launches the autotuning moved to the kernels with double-width K tiles.
- * At most 21 entries are written. */
+ * launches the autotuning moved to the kernels with double-width K tiles,
+ * [21] 1 when the last refinement took back a correction that raised the
+ * worst residual (the iterate before it is the result), [22] 1 when the last
+ * solve of a mixed-precision handle missed the refinement target and was
+ * repeated with an FP64 factorization of the same values (built at the first
+ * such solve, kept until the next refactorization), [23] the relative
+ * residual that repeat reached (-1 if none ran).
+ * At most 24 entries are written. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */
@@ -162,7 +168,18 @@ typedef struct rustyfem_pcg_options {
                            omega = 4/(3 rho) (src/solver/iterative/amg.rs:300) */
   int32_t degree;       /* smoothing steps before and after the coarse correction; 0 = 2 */
   double cheb_ratio;    /* 0 = 20 */
+  int32_t flags;        /* RUSTYFEM_PCG_* bits below; 0 = IterMatrix's behaviour */
+  int32_t reserved;
 } rustyfem_pcg_options;
+
+/* flags: keep the host AMG hierarchy for rustyfem_pcg_level */
+#define RUSTYFEM_PCG_KEEP_HIERARCHY 1
+/* flags: no direct fallback (solve_lower's behaviour, src/solver/iterative.rs:275-292:
+ * a non-converged solve is reported, not repeated directly) */
+#define RUSTYFEM_PCG_NO_DIRECT_FALLBACK 2
+/* flags, test hook: the AMG setup reports a breakdown as if the coarse LDL^T had
+ * failed, so that the direct fallback can be exercised on a healthy matrix */
+#define RUSTYFEM_PCG_FORCE_AMG_BREAKDOWN 4
 
 typedef struct PcgHandle PcgHandle;
 
@@ -177,6 +194,21 @@ typedef struct PcgHandle PcgHandle;
 PcgHandle *rustyfem_pcg_setup(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,
                               int32_t kind, size_t nn, const double *coords, const int64_t *ndof,
                               const rustyfem_pcg_options *opt);
+
+/* rustyfem_pcg_setup with IterMatrix::factor's error report
+ * (src/solver/iterative.rs:208-243): a non-positive diagonal entry sends the
+ * matrix to the direct solver for diagnosis, and so does an AMG hierarchy
+ * that cannot be built.  When the direct factorization (positive definite
+ * multifrontal LDL^T, nested dissection with 48-node leaves as
+ * src/sparse.rs:14) succeeds the handle solves directly; when it fails the
+ * result is NULL with *err_eq = the natural equation of the failing pivot
+ * (-1 for other errors).  Without RUSTYFEM_PCG_NO_DIRECT_FALLBACK a solve
+ * that does not converge is repeated with the direct solver, as
+ * IterMatrix::solve does (src/solver/iterative.rs:249-269);
+ * rustyfem_pcg_stats out[40..42] report which fallback ran. */
+PcgHandle *rustyfem_pcg_setup_ex(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,
+                                 int32_t kind, size_t nn, const double *coords, const int64_t *ndof,
+                                 const rustyfem_pcg_options *opt, int64_t *err_eq);
 
 /* Solves from x0 = 0 until the recursive residual satisfies ||r|| <= tol ||b||
  * (src/solver/iterative/csr.rs:341-380).  b and x are host pointers, or device
@@ -195,8 +227,34 @@ int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol,
  * smoother set; else -1), [26] iterations of the abandoned learned-smoother
  * attempt, [27] learned-smoother sweeps (0 = none), [28..39] DOFs per
  * block of the level matrices, finest first (6 x 6 blocks for the aggregates
- * of a 3D hierarchy; 1 = scalar CSR). */
+ * of a 3D hierarchy; 1 = scalar CSR), [40] the handle's direct solver: 0 none,
+ * 1 set up because a diagonal entry is not positive, 2 because the AMG
+ * hierarchy broke down, 3 built when a solve did not converge, [41] 1 when the
+ * last solve's result came from the direct solver, [42] the true relative
+ * residual ||b - A x|| / ||b|| of that direct solution (-1 otherwise). */
 void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
+
+/* Host copy of AMG level `level` of a handle set up with
+ * RUSTYFEM_PCG_KEEP_HIERARCHY (additive, for parity checks; the arrays of
+ * src/solver/iterative/amg.rs:282-341).  Levels 0 .. L-1 are the smoothed
+ * levels, level L the coarsest matrix (A and B only).  what:
+ *   -1: iout[0..7] = rows, nnz(A), blocks, aggregates, near-null-space columns,
+ *       coarse columns (rows of level + 1), nnz(P~), nnz(P)
+ *    0: A (CSR: iout = row pointers, idx = columns, dout = values)
+ *    1: P~, the tentative prolongator (CSR), 2: P, the smoothed one (CSR)
+ *    3: the aggregate of every block (iout), 4: block boundaries (iout, blocks + 1)
+ *    5: the near-null space B of this level (dout, rows x columns, row-major)
+ * Returns 0, or -1 when the level or the hierarchy does not exist. */
+int32_t rustyfem_pcg_level(const PcgHandle *h, int32_t level, int32_t what, int64_t *iout, int32_t *idx,
+                           double *dout);
+
+/* Vanek's three-pass aggregation of one level alone (additive, host only, for
+ * parity checks: src/solver/iterative/amg.rs:108-181), as the AMG setup runs
+ * it: a full symmetric CSR matrix (ptr n + 1, idx, val), DOF blocks
+ * [bptr[b], bptr[b + 1]) and the strength threshold theta.  agg receives the
+ * aggregate of every block; returns the number of aggregates (-1 on error). */
+int64_t rustyfem_amg_aggregate(size_t n, const int64_t *ptr, const int32_t *idx, const double *val, size_t nblk,
+                               const int64_t *bptr, double theta, int64_t *agg);
 
 /* Coefficients of the V-cycle's Chebyshev smoother (additive, no counterpart
  * in the reference, whose smoother is Gauss-Seidel): degree steps on

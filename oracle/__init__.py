@@ -88,6 +88,14 @@ def lib():
             "ora_rigid_body_modes": (sz, [sz, f64p, i64p, f64p]),
             "ora_set_threads": (None, [C.c_int]),
             "ora_threads": (C.c_int, []),
+            "ora_rsort_by_ikey": (None, [i64p, sz, vp]),
+            "ora_amg_aggregate": (C.c_int64, [sz, i64p, i64p, f64p, sz, i64p, C.c_double, i64p]),
+            "ora_amg_build": (vp, [sz, i64p, i64p, f64p, sz, f64p, i64p]),
+            "ora_amg_nlev": (sz, [vp]),
+            "ora_amg_level": (C.c_int32, [vp, sz, C.c_int32, vp, vp, vp]),
+            "ora_amg_free": (None, [vp]),
+            "ora_rsort_net9": (None, [i64p, vp]),
+            "ora_rsort_net13": (None, [i64p, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -107,6 +115,85 @@ def _f64(a):
 
 def set_threads(t: int) -> None:
     lib().ora_set_threads(int(t))
+
+
+def amg_aggregate(a, bptr, theta):
+    """The oracle's three-pass aggregation of one level (src/solver/iterative/amg.rs:108-181)
+    on a full symmetric scipy matrix: (aggregate per block, count)."""
+    import scipy.sparse as sp
+    m = sp.csr_matrix(a)
+    m.sort_indices()
+    bp = np.ascontiguousarray(bptr, dtype=np.int64)
+    agg = np.zeros(bp.size - 1, dtype=np.int64)
+    n = lib().ora_amg_aggregate(m.shape[0], np.ascontiguousarray(m.indptr, dtype=np.int64),
+                                np.ascontiguousarray(m.indices, dtype=np.int64),
+                                np.ascontiguousarray(m.data, dtype=np.float64), bp.size - 1, bp, float(theta), agg)
+    return agg, int(n)
+
+
+class AmgHierarchy:
+    """The oracle's Amg::new (src/solver/iterative/amg.rs:282-341) with every level recorded."""
+
+    def __init__(self, n, colptr, rowidx, val, coords, ndof):
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        self._h = lib().ora_amg_build(int(n), _i64(colptr), _i64(rowidx), _f64(val), c.shape[0], c,
+                                      _i64(ndof))
+        if not self._h:
+            raise RuntimeError("oracle AMG setup failed")
+        self.nlev = int(lib().ora_amg_nlev(self._h))
+
+    def level(self, l):
+        import scipy.sparse as sp
+        L = lib()
+        info = np.zeros(8, dtype=np.int64)
+        if L.ora_amg_level(self._h, int(l), -1, info.ctypes.data, None, None) != 0:
+            return None
+        rows, nnz_a, nblk, nagg, nbm, ncols, nnz_pt, nnz_p = (int(v) for v in info)
+
+        def csr(what, nr, nc, nnz):
+            ptr = np.zeros(nr + 1, dtype=np.int64)
+            idx = np.zeros(max(nnz, 1), dtype=np.int64)
+            val = np.zeros(max(nnz, 1))
+            L.ora_amg_level(self._h, int(l), what, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data)
+            return sp.csr_matrix((val[:nnz], idx[:nnz], ptr), shape=(nr, nc))
+
+        out = {"A": csr(0, rows, rows, nnz_a), "nblk": nblk, "nagg": nagg, "nbm": nbm}
+        bptr = np.zeros(nblk + 1, dtype=np.int64)
+        L.ora_amg_level(self._h, int(l), 4, bptr.ctypes.data, None, None)
+        out["bptr"] = bptr
+        B = np.zeros(max(rows * nbm, 1))
+        L.ora_amg_level(self._h, int(l), 5, None, None, B.ctypes.data)
+        out["B"] = B[:rows * nbm].reshape(rows, nbm)
+        if ncols > 0:
+            out["Pt"] = csr(1, rows, ncols, nnz_pt)
+            out["P"] = csr(2, rows, ncols, nnz_p)
+            agg = np.zeros(nblk, dtype=np.int64)
+            L.ora_amg_level(self._h, int(l), 3, agg.ctypes.data, None, None)
+            out["agg"] = agg
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_amg_free(self._h)
+            self._h = None
+
+
+def rust_sort_unstable(items, key=None):
+    """Items (int64) sorted by key[item] (by their own value when key is None)
+    with the restated unstable sort of Rust's core library (ora_rsort.c)."""
+    v = np.array(items, dtype=np.int64, copy=True)
+    k = None if key is None else np.ascontiguousarray(key, dtype=np.int64)
+    lib().ora_rsort_by_ikey(v, v.size, None if k is None else k.ctypes.data)
+    return v
+
+
+def rust_sort_network(items, key=None):
+    """One of the small sort's networks (9 or 13 inputs) applied alone."""
+    v = np.array(items, dtype=np.int64, copy=True)
+    k = None if key is None else np.ascontiguousarray(key, dtype=np.int64)
+    f = {9: lib().ora_rsort_net9, 13: lib().ora_rsort_net13}[v.size]
+    f(v, None if k is None else k.ctypes.data)
+    return v
 
 
 def threads() -> int:

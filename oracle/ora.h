@@ -12,13 +12,13 @@
  * library is compiled with -ffp-contract=off so that no other contraction
  * happens.
  *
- * One documented divergence: the reference sorts node sets by coordinate with
- * Rust's unstable sort (src/sparse.rs:108), whose tie order is an
- * implementation detail of the Rust standard library.  This oracle breaks
- * coordinate ties by the current position (a stable sort).  The tree shape,
- * the supernode sizes k, the front row counts u and every set-level result
- * are independent of the tie order; only the labels inside a supernode are
- * not (parity of within-supernode labels is unpinned).
+ * The reference sorts node sets by coordinate with Rust's unstable sort
+ * (src/sparse.rs:108), whose order among equal coordinates decides the labels
+ * inside a supernode.  ora_rsort.c restates that sort (ipnsort of Rust's core
+ * library, Rust >= 1.81) from its published algorithm; the restatement is
+ * pinned by properties and by the algorithm's fixed cases
+ * (tests/test_oracle_rsort.py), not by a run of the Rust binary, which cannot
+ * be built here.
  */
 #ifndef RUSTYFEM_ORACLE_H
 #define RUSTYFEM_ORACLE_H
@@ -140,6 +140,23 @@ size_t ora_rigid_body_modes(size_t nn, const double *coords, const int64_t *ndof
 /* threads used by the CPU-parallel parts (default hardware concurrency) */
 void ora_set_threads(int t);
 int ora_threads(void);
+
+/* ---- AMG hierarchy and aggregation hooks (ora_iter.c) ---- */
+typedef struct ora_amg ora_amg;
+int64_t ora_amg_aggregate(size_t n, const int64_t *ptr, const int64_t *idx, const double *val, size_t nblk,
+                          const int64_t *bptr, double theta, int64_t *agg_out);
+ora_amg *ora_amg_build(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, size_t nn,
+                       const double *coords, const int64_t *ndof);
+size_t ora_amg_nlev(const ora_amg *h);
+int32_t ora_amg_level(const ora_amg *h, size_t l, int32_t what, int64_t *iout, int64_t *idx, double *dout);
+void ora_amg_free(ora_amg *h);
+
+/* ---- test hooks of the restated Rust unstable sort (ora_rsort.c) ---- */
+/* items sorted by ikey[item] (ikey NULL: by the item value) */
+void ora_rsort_by_ikey(int64_t *items, size_t n, const int64_t *key);
+/* the 9- and 13-input sorting networks of its small sort, alone */
+void ora_rsort_net9(int64_t *v, const int64_t *ikey);
+void ora_rsort_net13(int64_t *v, const int64_t *ikey);
 
 #ifdef __cplusplus
 }

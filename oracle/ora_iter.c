@@ -21,6 +21,32 @@ static void csr_free(csr *a) {
 }
 static size_t csr_nnz(const csr *a) { return (size_t)a->ptr[a->nrows]; }
 
+static csr csr_copy(const csr *a) {
+  csr m;
+  size_t nnz = csr_nnz(a);
+  m.nrows = a->nrows;
+  m.ncols = a->ncols;
+  m.ptr = (int64_t *)ora_xmalloc((a->nrows + 1) * sizeof(int64_t));
+  m.idx = (int64_t *)ora_xmalloc((nnz + 1) * sizeof(int64_t));
+  m.val = (double *)ora_xmalloc((nnz + 1) * sizeof(double));
+  memcpy(m.ptr, a->ptr, (a->nrows + 1) * sizeof(int64_t));
+  memcpy(m.idx, a->idx, nnz * sizeof(int64_t));
+  memcpy(m.val, a->val, nnz * sizeof(double));
+  return m;
+}
+
+/* what amg_new built, level by level (for the hierarchy tests): A of levels
+ * 0..nlev (nlev = the coarsest), the aggregates, P~ and P of the smoothed
+ * levels, the block bounds and near-null spaces of every level */
+#define ORA_MAXLEV 13
+struct ora_amg {
+  size_t nlev, nbm;
+  csr A[ORA_MAXLEV], Pt[ORA_MAXLEV], P[ORA_MAXLEV];
+  int64_t *agg[ORA_MAXLEV], *bptr[ORA_MAXLEV];
+  size_t nblk[ORA_MAXLEV], nagg[ORA_MAXLEV];
+  double *B[ORA_MAXLEV];
+};
+
 /* src/solver/iterative/csr.rs:20-62 */
 static csr csr_from_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val) {
   csr m;
@@ -622,7 +648,7 @@ static csr sub_scaled(const csr *pt, const csr *ap, const double *diag, double o
 }
 
 /* src/solver/iterative/amg.rs:282-341; returns 0 on success, -1 when the coarse matrix breaks down */
-static int amg_new(precond *pc, csr a, int64_t *bptr, size_t nblk, double *b, size_t nbm) {
+static int amg_new(precond *pc, csr a, int64_t *bptr, size_t nblk, double *b, size_t nbm, ora_amg *rec) {
   size_t cap = 16;
   pc->A = (csr *)ora_xcalloc(cap, sizeof(csr));
   pc->P = (csr *)ora_xcalloc(cap, sizeof(csr));
@@ -649,6 +675,20 @@ static int amg_new(precond *pc, csr a, int64_t *bptr, size_t nblk, double *b, si
     int64_t *cptr = NULL;
     size_t ncb = 0;
     csr pt = tentative(bptr, nblk, agg, nagg, b, nbm, &bc, &cptr, &ncb);
+    if (rec && (double)pt.ncols <= 0.85 * (double)a.nrows) {
+      size_t l = pc->nlev;
+      rec->A[l] = csr_copy(&a);
+      rec->Pt[l] = csr_copy(&pt);
+      rec->agg[l] = (int64_t *)ora_xmalloc((nblk + 1) * sizeof(int64_t));
+      memcpy(rec->agg[l], agg, nblk * sizeof(int64_t));
+      rec->nagg[l] = nagg;
+      rec->nblk[l] = nblk;
+      rec->bptr[l] = (int64_t *)ora_xmalloc((nblk + 1) * sizeof(int64_t));
+      memcpy(rec->bptr[l], bptr, (nblk + 1) * sizeof(int64_t));
+      rec->B[l] = (double *)ora_xmalloc((a.nrows * nbm + 1) * sizeof(double));
+      memcpy(rec->B[l], b, a.nrows * nbm * sizeof(double));
+      rec->nbm = nbm;
+    }
     free(agg);
     if ((double)pt.ncols > 0.85 * (double)a.nrows) {
       csr_free(&pt);
@@ -668,6 +708,7 @@ static int amg_new(precond *pc, csr a, int64_t *bptr, size_t nblk, double *b, si
     csr_free(&apm);
     total += (double)csr_nnz(&ac);
     if (nsz < 16) pc->sizes[nsz++] = (int64_t)ac.nrows;
+    if (rec) rec->P[pc->nlev] = csr_copy(&p);
     pc->A[pc->nlev] = a;
     pc->P[pc->nlev] = p;
     pc->R[pc->nlev] = r;
@@ -682,6 +723,16 @@ static int amg_new(precond *pc, csr a, int64_t *bptr, size_t nblk, double *b, si
     theta *= 0.5;
   }
   for (size_t i = nsz; i < 16; i++) pc->sizes[i] = 0;
+  if (rec) {
+    size_t l = pc->nlev;
+    rec->nlev = l;
+    rec->A[l] = csr_copy(&a);
+    rec->nblk[l] = nblk;
+    rec->bptr[l] = (int64_t *)ora_xmalloc((nblk + 1) * sizeof(int64_t));
+    memcpy(rec->bptr[l], bptr, (nblk + 1) * sizeof(int64_t));
+    rec->B[l] = (double *)ora_xmalloc((a.nrows * nbm + 1) * sizeof(double));
+    memcpy(rec->B[l], b, a.nrows * nbm * sizeof(double));
+  }
   if (own_b) { free(b); free(bptr); }
   size_t n = a.nrows;
   double *l = (double *)ora_xcalloc(n * n + 1, sizeof(double));
@@ -901,7 +952,7 @@ int64_t ora_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, 
     memcpy(acopy.ptr, a.ptr, (n + 1) * sizeof(int64_t));
     memcpy(acopy.idx, a.idx, nnz * sizeof(int64_t));
     memcpy(acopy.val, a.val, nnz * sizeof(double));
-    if (amg_new(&pc, acopy, bptr, nn, bm, nbm) != 0) {
+    if (amg_new(&pc, acopy, bptr, nn, bm, nbm, NULL) != 0) {
       free(bptr);
       free(bm);
       precond_free(&pc);
@@ -1002,4 +1053,103 @@ size_t ora_natural_pattern(size_t nn, const int64_t *adj_ptr, const int64_t *adj
   vi_free(&nb);
   free(fst);
   return nnz;
+}
+
+/* ---- AMG hierarchy and aggregation hooks (for the tests) ---- */
+
+/* src/solver/iterative/amg.rs:108-181 alone, on a full CSR matrix */
+int64_t ora_amg_aggregate(size_t n, const int64_t *ptr, const int64_t *idx, const double *val, size_t nblk,
+                          const int64_t *bptr, double theta, int64_t *agg_out) {
+  csr a;
+  a.nrows = a.ncols = n;
+  a.ptr = (int64_t *)ptr;
+  a.idx = (int64_t *)idx;
+  a.val = (double *)val;
+  size_t nagg = 0;
+  int64_t *agg = aggregate(&a, bptr, nblk, theta, &nagg);
+  memcpy(agg_out, agg, nblk * sizeof(int64_t));
+  free(agg);
+  return (int64_t)nagg;
+}
+
+/* Amg::new (src/solver/iterative/amg.rs:282-341) on the lower triangle with
+ * rigid body modes, recording every level; NULL when the setup fails */
+ora_amg *ora_amg_build(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val, size_t nn,
+                       const double *coords, const int64_t *ndof) {
+  csr a = csr_from_lower(n, colptr, rowidx, val);
+  int64_t *bptr = (int64_t *)ora_xmalloc((nn + 1) * sizeof(int64_t));
+  bptr[0] = 0;
+  for (size_t v = 0; v < nn; v++) bptr[v + 1] = bptr[v] + ndof[v];
+  double *bm = (double *)ora_xmalloc((n * 6 + 1) * sizeof(double));
+  size_t nbm = ora_rigid_body_modes(nn, coords, ndof, bm);
+  precond pc;
+  memset(&pc, 0, sizeof(pc));
+  pc.kind = 0;
+  ora_amg *rec = (ora_amg *)ora_xcalloc(1, sizeof(ora_amg));
+  int rc = amg_new(&pc, a, bptr, nn, bm, nbm, rec);
+  free(bptr);
+  free(bm);
+  precond_free(&pc);
+  if (rc != 0) {
+    ora_amg_free(rec);
+    return NULL;
+  }
+  return rec;
+}
+
+size_t ora_amg_nlev(const ora_amg *h) { return h->nlev; }
+
+/* level l (0..nlev): what -1 info[0..7] = rows, nnz(A), blocks, aggregates,
+ * null-space columns, coarse columns, nnz(P~), nnz(P); 0 A, 1 P~, 2 P (CSR
+ * into iout/idx/dout); 3 aggregates, 4 block bounds (iout); 5 B (dout) */
+int32_t ora_amg_level(const ora_amg *h, size_t l, int32_t what, int64_t *iout, int64_t *idx, double *dout) {
+  if (l > h->nlev) return -1;
+  const int coarsest = l == h->nlev;
+  const csr *A = &h->A[l];
+  const csr *m = NULL;
+  switch (what) {
+    case -1: {
+      int64_t info[8] = {(int64_t)A->nrows, (int64_t)csr_nnz(A), (int64_t)h->nblk[l], 0, (int64_t)h->nbm, 0, 0, 0};
+      if (!coarsest) {
+        info[3] = (int64_t)h->nagg[l];
+        info[5] = (int64_t)h->P[l].ncols;
+        info[6] = (int64_t)csr_nnz(&h->Pt[l]);
+        info[7] = (int64_t)csr_nnz(&h->P[l]);
+      }
+      memcpy(iout, info, sizeof info);
+      return 0;
+    }
+    case 0: m = A; break;
+    case 1: if (coarsest) return -1; m = &h->Pt[l]; break;
+    case 2: if (coarsest) return -1; m = &h->P[l]; break;
+    case 3:
+      if (coarsest) return -1;
+      memcpy(iout, h->agg[l], h->nblk[l] * sizeof(int64_t));
+      return 0;
+    case 4:
+      memcpy(iout, h->bptr[l], (h->nblk[l] + 1) * sizeof(int64_t));
+      return 0;
+    case 5:
+      memcpy(dout, h->B[l], A->nrows * h->nbm * sizeof(double));
+      return 0;
+    default: return -1;
+  }
+  size_t nnz = csr_nnz(m);
+  if (iout) memcpy(iout, m->ptr, (m->nrows + 1) * sizeof(int64_t));
+  if (idx) memcpy(idx, m->idx, nnz * sizeof(int64_t));
+  if (dout) memcpy(dout, m->val, nnz * sizeof(double));
+  return 0;
+}
+
+void ora_amg_free(ora_amg *h) {
+  if (!h) return;
+  for (size_t l = 0; l < ORA_MAXLEV; l++) {
+    if (h->A[l].ptr) csr_free(&h->A[l]);
+    if (h->Pt[l].ptr) csr_free(&h->Pt[l]);
+    if (h->P[l].ptr) csr_free(&h->P[l]);
+    free(h->agg[l]);
+    free(h->bptr[l]);
+    free(h->B[l]);
+  }
+  free(h);
 }

@@ -222,7 +222,7 @@ static int64_t dissect(dctx *c, int64_t *set, size_t n) {
   for (int ai = 0; ai < 3; ai++) {
     int ax = axes[ai];
     if (!(hi[ax] > lo[ax])) continue;
-    ora_sort_by_dkey(set, n, X, 3, (size_t)ax);
+    ora_rsort_by_dkey(set, n, X, 3, (size_t)ax); /* sort_unstable_by, src/sparse.rs:108 */
     size_t qs[3] = {n / 2, 2 * n / 5, 3 * n / 5};
     for (int qi = 0; qi < 3; qi++) {
       double t = X[set[qs[qi]] * 3 + ax];
@@ -371,7 +371,7 @@ ora_sym *ora_analyse(size_t nn, const double *coords, const int64_t *adj_ptr, co
         if (S->first_dof[w] >= end && stamp[w] != ti) { stamp[w] = ti; vi_push(&list, w); }
       }
     }
-    ora_sort_by_ikey(list.a, list.n, S->first_dof);
+    ora_rsort_by_ikey(list.a, list.n, S->first_dof); /* sort_unstable_by_key, src/sparse.rs:224 */
     for (size_t i = 0; i < list.n; i++) {
       int64_t w = list.a[i];
       for (int64_t d = 0; d < ndof[w]; d++) vi_push(&t->rows, S->first_dof[w] + d);
@@ -447,7 +447,7 @@ size_t ora_sym_pattern(const ora_sym *s, const int64_t *adj_ptr, const int64_t *
   size_t nn = s->nn;
   int64_t *order = (int64_t *)ora_xmalloc((nn ? nn : 1) * sizeof(int64_t));
   for (size_t i = 0; i < nn; i++) order[i] = (int64_t)i;
-  ora_sort_by_ikey(order, nn, s->first_dof);
+  ora_rsort_by_ikey(order, nn, s->first_dof); /* src/sparse.rs:260 */
   size_t nnz = 0;
   if (colptr) colptr[0] = 0;
   vi nb = {0};
@@ -456,7 +456,7 @@ size_t ora_sym_pattern(const ora_sym *s, const int64_t *adj_ptr, const int64_t *
     nb.n = 0;
     for (int64_t p = adj_ptr[v]; p < adj_ptr[v + 1]; p++) vi_push(&nb, adj_idx[p]);
     vi_push(&nb, v);
-    ora_sort_by_ikey(nb.a, nb.n, s->first_dof);
+    ora_rsort_by_ikey(nb.a, nb.n, s->first_dof); /* src/sparse.rs:263 */
     int64_t f = s->first_dof[v];
     for (int64_t c = f; c < f + ndof[v]; c++) {
       for (size_t j = 0; j < nb.n; j++) {

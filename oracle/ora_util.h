@@ -42,6 +42,10 @@ void ora_sort_by_dkey(int64_t *items, size_t n, const double *key, size_t key_st
 void ora_sort_by_ikey(int64_t *items, size_t n, const int64_t *key);
 /* sort int64 ascending */
 void ora_sort_i64(int64_t *a, size_t n);
+/* Rust's unstable sort (ipnsort) restated, ora_rsort.c: the tie order of
+ * slice::sort_unstable_by on usize items */
+void ora_rsort_by_dkey(int64_t *items, size_t n, const double *key, size_t key_stride, size_t key_off);
+void ora_rsort_by_ikey(int64_t *items, size_t n, const int64_t *key);
 size_t ora_dedup_i64(int64_t *a, size_t n);
 /* binary search of x in sorted a[0..n); returns index or -1 */
 static inline int64_t ora_bsearch(const int64_t *a, size_t n, int64_t x) {

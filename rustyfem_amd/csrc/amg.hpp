@@ -11,10 +11,17 @@
 #pragma once
 #include <cstdint>
 #include <functional>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 namespace rf {
+
+// The hierarchy cannot be built: a level diagonal is not positive or the
+// coarse LDL^T breaks down (the Err(()) of src/solver/iterative/amg.rs:282-341).
+struct AmgBreakdown : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 // scalar CSR with sorted column indices
 struct Csr {
@@ -41,6 +48,11 @@ struct AmgHostLevel {
   double omega = 0.0;
   int64_t nblk = 0;     // node blocks of this level
   std::vector<int64_t> bptr;
+  // kept only when amg_setup is asked to keep the hierarchy (parity checks):
+  std::vector<int64_t> agg;  // aggregate of every block
+  Csr Pt;                    // tentative prolongator
+  std::vector<double> B;     // near-null space of this level (rows x nbm, row-major)
+  int nbm = 0;
 };
 
 struct AmgHost {
@@ -50,14 +62,18 @@ struct AmgHost {
   std::vector<double> coarse_l, coarse_d;  // its dense LDL^T (unit L row-major, D)
   std::vector<int64_t> sizes;         // rows per level, finest first, coarsest last
   double complexity = 0.0;            // total nnz / fine nnz
+  std::vector<double> coarse_B;       // near-null space of the coarsest level (kept hierarchy only)
 };
 
 // Builds the hierarchy (src/solver/iterative/amg.rs:282-341).  spectral_radius
 // returns 1.05 x the 15-step power-method estimate of rho(D^-1 A) for a level.
 // Throws when the coarse LDL^T breaks down (d <= 1e-13 dmax) or a diagonal is
 // not positive.
+// keep = true also records the aggregates, the tentative prolongators and the
+// near-null spaces of every level (AmgHostLevel::agg, Pt, B; coarse_B).
 AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int nb,
-                  const std::function<double(const Csr&, const std::vector<double>&)>& spectral_radius);
+                  const std::function<double(const Csr&, const std::vector<double>&)>& spectral_radius,
+                  bool keep = false);
 
 // A_c^-1 = L^-T D^-1 L^-1 as a dense row-major matrix, from the coarse LDL^T
 std::vector<double> coarse_inverse(const std::vector<double>& l, const std::vector<double>& d, int64_t n);

@@ -403,7 +403,7 @@ Csr sub_scaled(const Csr& pt, const Csr& ap, const std::vector<double>& diag, do
 }  // namespace
 
 AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int nb,
-                  const std::function<double(const Csr&, const std::vector<double>&)>& spectral_radius) {
+                  const std::function<double(const Csr&, const std::vector<double>&)>& spectral_radius, bool keep) {
   AmgHost H;
   H.sizes.push_back(a.nrows);
   const double fine_nnz = static_cast<double>(a.nnz());
@@ -412,7 +412,7 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
   while (a.nrows > 1500 && H.levels.size() < 12) {
     std::vector<double> diag = csr_diag(a);
     for (double d : diag)
-      if (!(d > 0.0)) throw std::runtime_error("AMG setup: a diagonal entry is not positive");
+      if (!(d > 0.0)) throw AmgBreakdown("AMG setup: a diagonal entry is not positive");
     int64_t nagg = 0;
     const std::vector<int64_t> agg = amg_aggregate(a, bptr, theta, nagg);
     std::vector<double> bc;
@@ -424,12 +424,18 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
     Csr ap = csr_mul(a, pt);
     Csr p = sub_scaled(pt, ap, diag, omega);
     ap = Csr();
+    AmgHostLevel L;
+    if (keep) {
+      L.agg = agg;
+      L.Pt = std::move(pt);
+      L.B = b;
+      L.nbm = nb;
+    }
     pt = Csr();
     Csr r = csr_transpose(p);
     Csr ac = csr_mul(r, csr_mul(a, p));
     total += static_cast<double>(ac.nnz());
     H.sizes.push_back(ac.nrows);
-    AmgHostLevel L;
     L.nblk = static_cast<int64_t>(bptr.size()) - 1;
     L.bptr = std::move(bptr);
     L.A = std::move(a);
@@ -445,6 +451,7 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
     theta *= 0.5;
   }
   H.coarse_bptr = bptr;
+  if (keep) H.coarse_B = b;
   // dense LDL^T of the coarsest matrix (amg.rs:319-340)
   const int64_t n = a.nrows;
   std::vector<double> l(static_cast<size_t>(n) * n, 0.0), d(n, 0.0);
@@ -455,7 +462,7 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
   for (int64_t j = 0; j < n; ++j) {
     double dj = l[j * n + j];
     for (int64_t k = 0; k < j; ++k) dj -= l[j * n + k] * l[j * n + k] * d[k];
-    if (!(dj > 1e-13 * dmax)) throw std::runtime_error("AMG setup: the coarse LDL^T broke down");
+    if (!(dj > 1e-13 * dmax)) throw AmgBreakdown("AMG setup: the coarse LDL^T broke down");
     d[j] = dj;
     for (int64_t i = j + 1; i < n; ++i) {
       double s = l[i * n + j];

@@ -6,6 +6,7 @@
 // the GPU (mf_gpu.hip, indef_gpu.hip).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../include/rustyfem.h"
+#include "amg.hpp"
 #include "gpu.hpp"
 #include "indef.hpp"
 #include "mf.hpp"
@@ -65,8 +67,11 @@ struct Handle {
   std::vector<int64_t> colptr, rowidx;
   std::vector<int64_t> owner;
   std::unique_ptr<rf::GpuFactor> gpu;
+  // mixed precision: the FP64 factor of the same values, built when a solve's
+  // refinement misses its target, dropped at the next refactorization
+  std::unique_ptr<rf::GpuFactor> gpu64;
   std::unique_ptr<rf::IndefGpu> ind;
-  std::unique_ptr<rf::Symbolic> sym;  // kept only by host-only analyses
+  std::unique_ptr<rf::Symbolic> sym;  // kept by host-only analyses and mixed-precision handles
   // solves share the handle's device work buffers and graphs: concurrent
   // rustyfem_solve calls on one handle (allowed by ffi/src/lib.rs:336-371)
   // are serialized here
@@ -208,6 +213,22 @@ Handle* build(size_t n, size_t nn, const int64_t* loc, const double* coords, con
     h->gpu->set_caller_map(h->eq, h->diag);
     const char* kt = std::getenv("RUSTYFEM_KERNEL_TIMING");
     h->gpu->set_kernel_timing(kt && *kt && *kt != '0');
+    if (opt.prec == rf::Precision::MIXED) {
+      h->sym = std::make_unique<rf::Symbolic>(std::move(sym));
+      Handle* hp = h.get();
+      // refinement of the FP32 factor missed its target: the FP64 factor of
+      // the same values (built once, on the values in device memory) solves
+      // the same internal right-hand sides with the same refinement
+      h->gpu->set_fallback([hp](const double* d_b, double* d_x, int nrhs) -> double {
+        if (!hp->gpu64) {
+          rf::Plan p64 = rf::build_plan(*hp->sym, hp->colptr, hp->rowidx);
+          auto g = rf::make_gpu_factor(std::move(p64), hp->colptr, hp->rowidx, rf::Precision::FP64);
+          if (g->factor(hp->gpu->device_lower_values(), true) >= 0) return -1.0;
+          hp->gpu64 = std::move(g);
+        }
+        return hp->gpu64->solve_internal(d_b, d_x, nrhs, hp->opt.refine_tol, std::max(hp->opt.max_refine, 1));
+      });
+    }
   } else {
     h->ind = rf::make_indef_gpu(std::move(sym), h->colptr, h->rowidx, mode == Mode::Lu);
     h->ind->set_caller_map(h->eq, h->diag);
@@ -227,6 +248,7 @@ std::vector<double> lower_values(const Handle* h, const double* data) {
 // numeric factorization; returns -1 or the caller DOF of the failing pivot
 int64_t numeric(Handle* h, const double* data) {
   if (h->mode == Mode::Pd) {
+    h->gpu64.reset();
     int64_t e;
     if (data) {
       const std::vector<double> v = lower_values(h, data);
@@ -360,7 +382,7 @@ bool batch_numeric(BatchHandle* h, const double* const* data, int64_t* err) {
 
 // entries of rustyfem_stats / rustyfem_batch_stats that come from the
 // factorization's timings (the layout is documented in rustyfem.h)
-constexpr size_t NSTATS = 21;
+constexpr size_t NSTATS = 24;
 void fill_timings(const rf::Timings& t, double* v) {
   v[0] = t.factor_ms;
   v[1] = t.solve_ms;
@@ -375,6 +397,9 @@ void fill_timings(const rf::Timings& t, double* v) {
   v[15] = t.agpr_launches;
   for (int i = 0; i < 4; ++i) v[16 + i] = t.refine_history[i];
   v[20] = t.ilv_launches;
+  v[21] = t.refine_undone;
+  v[22] = t.fell_back;
+  v[23] = t.fallback_rel;
 }
 
 }  // namespace
@@ -631,14 +656,113 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
 
 struct PcgHandle {
-  std::unique_ptr<rf::PcgGpu> solver;
+  std::unique_ptr<rf::PcgGpu> solver;  // null when the handle solves directly
   int64_t n = 0;
   std::mutex mu;
+  // IterMatrix's direct solver (src/solver/iterative.rs:138-190): positive
+  // definite multifrontal LDL^T of the same values, ND leaves of 48 nodes
+  std::unique_ptr<Handle> direct;
+  int direct_reason = 0;  // 0 none, 1 non-positive diagonal, 2 AMG breakdown, 3 non-convergence
+  bool fallback = true;   // repeat non-converged solves directly
+  bool last_direct = false;
+  double last_direct_rel = -1.0;
+  // the inputs, for a direct factor built when a solve does not converge
+  std::vector<int64_t> colptr, rowidx, ndof;
+  std::vector<double> val, coords;
+  int64_t nn = 0;
 };
 
-PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int32_t kind,
-                              size_t nn, const double* coords, const int64_t* ndof, const rustyfem_pcg_options* opt) {
+namespace {
+
+// The lower triangle (natural numbering) as a caller matrix of the PD
+// factorization: both triangles, loc from ndof.  Returns the handle, or
+// nullptr with *err = the natural equation of the failing pivot (-1 when the
+// input is rejected otherwise).
+Handle* direct_factor(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int64_t nn,
+                      const double* coords, const int64_t* ndof, int64_t* err) {
+  std::vector<int64_t> loc(n, 0);
+  for (int64_t v = 0, i = 0; v < nn; ++v)
+    for (int64_t l = 0; l < ndof[v]; ++l) loc[i++] = v;
+  std::vector<int64_t> cnt(n + 1, 0);
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p) {
+      ++cnt[c + 1];
+      if (rowidx[p] != c) ++cnt[rowidx[p] + 1];
+    }
+  for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1), ind(cnt[n]);
+  std::vector<double> dat(cnt[n]);
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p) {
+      const int64_t r = rowidx[p];
+      ind[pos[c]] = r;
+      dat[pos[c]++] = val[p];
+      if (r != c) {
+        ind[pos[r]] = c;
+        dat[pos[r]++] = val[p];
+      }
+    }
+  Options opt;
+  opt.leaf = 48;  // sparse::analyse, src/sparse.rs:14
+  return factor_mode(static_cast<size_t>(n), static_cast<size_t>(nn), loc.data(), coords, cnt.data(), ind.data(),
+                     dat.data(), Mode::Pd, opt, nullptr, err);
+}
+
+// ||b - A x|| / ||b|| of host vectors for the lower triangle in CSC
+double lower_rel_residual(const PcgHandle* h, const double* b, const double* x) {
+  const int64_t n = h->n;
+  std::vector<double> r(b, b + n);
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t p = h->colptr[c]; p < h->colptr[c + 1]; ++p) {
+      const int64_t q = h->rowidx[p];
+      r[q] -= h->val[p] * x[c];
+      if (q != c) r[c] -= h->val[p] * x[q];
+    }
+  double rn = 0.0, bn = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    rn += r[i] * r[i];
+    bn += b[i] * b[i];
+  }
+  return bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+}
+
+// x = A^-1 b with the direct factor (host or device vectors); returns the
+// true relative residual, or NaN with x = NaN when no direct factor exists
+// (IterMatrix::solve_direct, src/solver/iterative.rs:167-190)
+double solve_directly(PcgHandle* h, const double* b, double* x, bool on_device) {
+  const int64_t n = h->n;
+  std::vector<double> hb(n), hx(n);
+  if (on_device) rf::copy_to_host(hb.data(), b, n * sizeof(double));
+  else std::memcpy(hb.data(), b, n * sizeof(double));
+  if (!h->direct) {
+    int64_t e = -1;
+    h->direct.reset(direct_factor(n, h->colptr.data(), h->rowidx.data(), h->val.data(), h->nn, h->coords.data(),
+                                  h->ndof.data(), &e));
+    if (!h->direct) {
+      std::fprintf(stderr, "warning: the direct fallback solution failed (singular matrix)\n");
+      std::fill(hx.begin(), hx.end(), std::nan(""));
+      if (on_device) rf::copy_to_device(x, hx.data(), n * sizeof(double));
+      else std::memcpy(x, hx.data(), n * sizeof(double));
+      return std::nan("");
+    }
+    if (!h->direct_reason) h->direct_reason = 3;
+  }
+  hx = hb;
+  if (rustyfem_solve(h->direct.get(), 1, hx.data()) != 0) throw std::runtime_error("direct solve failed: " + rf::last_error());
+  if (on_device) rf::copy_to_device(x, hx.data(), n * sizeof(double));
+  else std::memcpy(x, hx.data(), n * sizeof(double));
+  return lower_rel_residual(h, hb.data(), hx.data());
+}
+
+}  // namespace
+
+PcgHandle* rustyfem_pcg_setup_ex(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
+                                 int32_t kind, size_t nn, const double* coords, const int64_t* ndof,
+                                 const rustyfem_pcg_options* opt, int64_t* err_eq) {
   set_error("");
+  int64_t dummy = -1;
+  if (!err_eq) err_eq = &dummy;
+  *err_eq = -1;
   try {
     // the lower triangle in CSC, natural numbering (src/solver/iterative.rs:74-97)
     for (size_t c = 0; c < n; ++c) {
@@ -654,6 +778,7 @@ PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* ro
     }
     if (kind == 0 && tot != static_cast<int64_t>(n)) throw std::runtime_error("sum of ndof differs from n");
     rf::PcgOptions o;
+    const int32_t flags = opt ? opt->flags : 0;
     if (opt) {
       o.krylov = opt->krylov;
       o.vcycle_fp32 = opt->vcycle_fp32;
@@ -661,10 +786,50 @@ PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* ro
       if (opt->degree > 0) o.degree = opt->degree;
       if (opt->cheb_ratio > 1.0) o.cheb_ratio = opt->cheb_ratio;
     }
+    o.keep_hierarchy = (flags & RUSTYFEM_PCG_KEEP_HIERARCHY) != 0;
+    o.force_amg_breakdown = (flags & RUSTYFEM_PCG_FORCE_AMG_BREAKDOWN) != 0;
     auto h = std::make_unique<PcgHandle>();
     h->n = static_cast<int64_t>(n);
-    h->solver = rf::make_pcg_gpu(static_cast<int64_t>(n), colptr, rowidx, val, kind, static_cast<int64_t>(nn), coords,
-                                 ndof, o);
+    h->fallback = (flags & RUSTYFEM_PCG_NO_DIRECT_FALLBACK) == 0;
+    if (h->fallback) {
+      h->colptr.assign(colptr, colptr + n + 1);
+      h->rowidx.assign(rowidx, rowidx + colptr[n]);
+      h->val.assign(val, val + colptr[n]);
+      h->nn = static_cast<int64_t>(nn);
+      h->ndof.assign(ndof, ndof + nn);
+      h->coords.assign(coords, coords + 3 * nn);
+      // a diagonal entry that is not positive: diagnosis by the direct solver
+      // (IterMatrix::factor, src/solver/iterative.rs:211-215)
+      bool bad = false;
+      for (size_t c = 0; c < n && !bad; ++c)
+        bad = !(colptr[c] < colptr[c + 1] && rowidx[colptr[c]] == static_cast<int64_t>(c) && val[colptr[c]] > 0.0);
+      if (bad) {
+        int64_t e = -1;
+        h->direct.reset(direct_factor(h->n, colptr, rowidx, val, h->nn, coords, ndof, &e));
+        if (!h->direct) {
+          *err_eq = e;
+          if (rf::last_error().empty()) set_error("a diagonal entry is not positive; the direct factorization failed");
+          return nullptr;
+        }
+        h->direct_reason = 1;
+        return h.release();
+      }
+    }
+    try {
+      h->solver = rf::make_pcg_gpu(static_cast<int64_t>(n), colptr, rowidx, val, kind, static_cast<int64_t>(nn),
+                                   coords, ndof, o);
+    } catch (const rf::AmgBreakdown& ex) {
+      if (!h->fallback) throw;
+      // AMG could not be built: direct, with diagnosis (src/solver/iterative.rs:237-241)
+      int64_t e = -1;
+      h->direct.reset(direct_factor(h->n, colptr, rowidx, val, h->nn, coords, ndof, &e));
+      if (!h->direct) {
+        *err_eq = e;
+        set_error(std::string(ex.what()) + "; the direct factorization failed");
+        return nullptr;
+      }
+      h->direct_reason = 2;
+    }
     return h.release();
   } catch (const std::exception& ex) {
     set_error(ex.what());
@@ -674,14 +839,39 @@ PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* ro
   return nullptr;
 }
 
+PcgHandle* rustyfem_pcg_setup(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int32_t kind,
+                              size_t nn, const double* coords, const int64_t* ndof, const rustyfem_pcg_options* opt) {
+  return rustyfem_pcg_setup_ex(n, colptr, rowidx, val, kind, nn, coords, ndof, opt, nullptr);
+}
+
 int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol, int64_t maxit, int32_t on_device,
                            double* rel_residual, int32_t* converged) {
   try {
     std::lock_guard<std::mutex> lock(h->mu);
+    h->last_direct = false;
+    h->last_direct_rel = -1.0;
+    if (!h->solver) {
+      const double rel = solve_directly(h, b, x, on_device != 0);
+      h->last_direct = true;
+      h->last_direct_rel = rel;
+      if (rel_residual) *rel_residual = rel;
+      if (converged) *converged = std::isfinite(rel) ? 1 : 0;
+      return 0;
+    }
     if (maxit <= 0) maxit = std::max<int64_t>(20000, h->n);
     const rf::PcgResult r = h->solver->solve(b, x, tol, maxit, on_device != 0);
     if (rel_residual) *rel_residual = r.rel_residual;
     if (converged) *converged = r.converged ? 1 : 0;
+    if (!r.converged && h->fallback) {
+      // not converged: solving directly (src/solver/iterative.rs:262-268)
+      std::fprintf(stderr, "warning: PCG did not converge (%lld iterations, relative residual %.2e); solving directly\n",
+                   static_cast<long long>(r.iterations), r.rel_residual);
+      const double rel = solve_directly(h, b, x, on_device != 0);
+      h->last_direct = true;
+      h->last_direct_rel = rel;
+      if (rel_residual) *rel_residual = rel;
+      if (converged) *converged = std::isfinite(rel) ? 1 : 0;
+    }
     return r.iterations;
   } catch (const std::exception& ex) {
     set_error(ex.what());
@@ -691,7 +881,96 @@ int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol,
   return -1;
 }
 
-void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) { h->solver->stats(out, len); }
+void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) {
+  double v[43];
+  for (double& x : v) x = 0.0;
+  if (h->solver) h->solver->stats(v, 40);
+  v[25] = h->solver ? v[25] : -1.0;
+  v[40] = h->direct_reason;
+  v[41] = h->last_direct ? 1.0 : 0.0;
+  v[42] = h->last_direct_rel;
+  for (size_t i = 0; i < len && i < 43; ++i) out[i] = v[i];
+}
+
+int32_t rustyfem_pcg_level(const PcgHandle* h, int32_t level, int32_t what, int64_t* iout, int32_t* idx,
+                           double* dout) {
+  try {
+    const rf::AmgHost* H = h->solver ? h->solver->hierarchy() : nullptr;
+    if (!H || level < 0 || level > static_cast<int32_t>(H->levels.size())) return -1;
+    const bool coarsest = level == static_cast<int32_t>(H->levels.size());
+    const rf::Csr& A = coarsest ? H->coarse : H->levels[level].A;
+    const std::vector<int64_t>& bptr = coarsest ? H->coarse_bptr : H->levels[level].bptr;
+    const std::vector<double>& B = coarsest ? H->coarse_B : H->levels[level].B;
+    const int64_t nbm = A.nrows > 0 ? static_cast<int64_t>(B.size()) / A.nrows : 0;
+    auto csr = [&](const rf::Csr& m) {
+      if (iout) std::copy(m.ptr.begin(), m.ptr.end(), iout);
+      if (idx) std::copy(m.idx.begin(), m.idx.end(), idx);
+      if (dout) std::copy(m.val.begin(), m.val.end(), dout);
+      return 0;
+    };
+    switch (what) {
+      case -1: {
+        int64_t info[8] = {A.nrows, A.nnz(), static_cast<int64_t>(bptr.size()) - 1, 0, nbm, 0, 0, 0};
+        if (!coarsest) {
+          const rf::AmgHostLevel& L = H->levels[level];
+          int64_t nagg = 0;
+          for (int64_t g : L.agg) nagg = std::max(nagg, g + 1);
+          info[3] = nagg;
+          info[5] = L.P.ncols;
+          info[6] = L.Pt.nnz();
+          info[7] = L.P.nnz();
+        }
+        if (iout) std::copy(info, info + 8, iout);
+        return 0;
+      }
+      case 0:
+        return csr(A);
+      case 1:
+        return coarsest ? -1 : csr(H->levels[level].Pt);
+      case 2:
+        return coarsest ? -1 : csr(H->levels[level].P);
+      case 3:
+        if (coarsest) return -1;
+        if (iout) std::copy(H->levels[level].agg.begin(), H->levels[level].agg.end(), iout);
+        return 0;
+      case 4:
+        if (iout) std::copy(bptr.begin(), bptr.end(), iout);
+        return 0;
+      case 5:
+        if (dout) std::copy(B.begin(), B.end(), dout);
+        return 0;
+      default:
+        return -1;
+    }
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int64_t rustyfem_amg_aggregate(size_t n, const int64_t* ptr, const int32_t* idx, const double* val, size_t nblk,
+                               const int64_t* bptr, double theta, int64_t* agg) {
+  try {
+    rf::Csr a;
+    a.nrows = a.ncols = static_cast<int64_t>(n);
+    a.ptr.assign(ptr, ptr + n + 1);
+    a.idx.assign(idx, idx + ptr[n]);
+    a.val.assign(val, val + ptr[n]);
+    const std::vector<int64_t> bp(bptr, bptr + nblk + 1);
+    if (bp.back() != static_cast<int64_t>(n)) throw std::runtime_error("blocks do not cover the rows");
+    int64_t nagg = 0;
+    const std::vector<int64_t> g = rf::amg_aggregate(a, bp, theta, nagg);
+    std::copy(g.begin(), g.end(), agg);
+    return nagg;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
 
 int32_t rustyfem_chebyshev(double hi, double ratio, int32_t degree, double* c0, double* c1, double* c2) {
   if (!(hi > 0.0) || !(ratio > 1.0) || degree < 1) return -1;
@@ -707,6 +986,7 @@ int32_t rustyfem_chebyshev(double hi, double ratio, int32_t degree, double* c0, 
 int32_t rustyfem_pcg_precondition(PcgHandle* h, const double* r, double* z, int32_t on_device) {
   try {
     std::lock_guard<std::mutex> lock(h->mu);
+    if (!h->solver) throw std::runtime_error("the handle solves directly: it has no preconditioner");
     h->solver->apply(r, z, on_device != 0);
     return 0;
   } catch (const std::exception& ex) {
@@ -721,6 +1001,7 @@ int32_t rustyfem_pcg_set_learned_smoother(PcgHandle* h, const double* params, si
   try {
     if (static_cast<int64_t>(nparams) != rf::mpnn_nparams()) throw std::runtime_error("wrong number of parameters");
     std::lock_guard<std::mutex> lock(h->mu);
+    if (!h->solver) throw std::runtime_error("the handle solves directly: it has no AMG hierarchy");
     h->solver->set_learned_smoother(params, sweeps);
     return 0;
   } catch (const std::exception& ex) {
@@ -790,7 +1071,10 @@ void rustyfem_pcg_free(PcgHandle* h) { delete h; }
 int64_t rustyfem_solve_lower(size_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
                              const double* b, int32_t kind, size_t nn, const double* coords, const int64_t* ndof,
                              double tol, int64_t maxit, double* x, double* rel_residual, int32_t* converged) {
-  std::unique_ptr<PcgHandle> h(rustyfem_pcg_setup(n, colptr, rowidx, val, kind, nn, coords, ndof, nullptr));
+  // solve_lower (src/solver/iterative.rs:275-292) has no direct fallback
+  rustyfem_pcg_options o{};
+  o.flags = RUSTYFEM_PCG_NO_DIRECT_FALLBACK;
+  std::unique_ptr<PcgHandle> h(rustyfem_pcg_setup(n, colptr, rowidx, val, kind, nn, coords, ndof, &o));
   if (!h) return -1;
   return rustyfem_pcg_solve(h.get(), b, x, tol, maxit, 0, rel_residual, converged);
 }

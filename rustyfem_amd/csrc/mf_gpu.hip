@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 
 #include "gemm_kernel.hpp"
@@ -1303,9 +1304,10 @@ __global__ __launch_bounds__(64) void k_sumsq_final(const double* __restrict__ p
   }
 }
 
-__global__ void k_axpy(double* __restrict__ x, const double* __restrict__ c, int64_t cnt) {
+// x += alpha c (alpha = 1: a plain sum; -1 takes a correction back)
+__global__ void k_axpy(double* __restrict__ x, const double* __restrict__ c, int64_t cnt, double alpha) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < cnt) x[i] += c[i];
+  if (i < cnt) x[i] = fma(alpha, c[i], x[i]);
 }
 
 // caller numbering <-> internal numbering (ffi/src/lib.rs:344-355)
@@ -1347,10 +1349,66 @@ X* dupload(const std::vector<X>& v, size_t& total, hipStream_t s) {
 
 inline unsigned grid(int64_t count) { return static_cast<unsigned>(count); }
 
+// Work vectors of one solve: front vectors t and ys (sum of m per column),
+// backward partial sums, x, b, two residual/correction vectors r and c, and
+// the norm reduction buffers.
+struct SolveWork {
+  double *t = nullptr, *ys = nullptr, *part = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *c = nullptr,
+         *red = nullptr, *nrm = nullptr;
+};
+
+// Scratch of the allocation-free solve (rustyfem_solve_device_noalloc): 64
+// MiB per device, held by every device the library runs on until the process
+// ends.  Host code never uses this name as an address (it is a host-side
+// stand-in); scratch_ptr() resolves the device address once per device.
+constexpr size_t kScratchElems = size_t{8} << 20;  // doubles (64 MiB)
+constexpr int kNoAllocMaxRhs = 64;
+constexpr int kMaxDevices = 64;
+
+// Makes dev current for the lifetime of the guard, then restores the caller's device.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    HIPCHK(hipGetDevice(&prev));
+    if (prev != dev) HIPCHK(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchElems];
+
+namespace {
+
+// Device address of `scratch` on device dev (which must be current): looked up
+// with hipGetSymbolAddress at the first call on that device and kept.
+double* scratch_ptr(int dev) {
+  static std::mutex mu;
+  static double* ptr[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("allocation-free solve: device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char m[200];
+      snprintf(m, sizeof m, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+      throw std::runtime_error(m);
+    }
+    ptr[dev] = static_cast<double*>(p);
+  }
+  return ptr[dev];
+}
+
 template <typename T>
 class Impl final : public GpuFactor {
  public:
   Impl(Plan plan, const std::vector<int64_t>& colptr, const std::vector<int64_t>& rowidx) : P_(std::move(plan)) {
+    HIPCHK(hipGetDevice(&dev_));
     HIPCHK(hipStreamCreate(&s_));
     HIPCHK(hipStreamCreate(&s2_));
     HIPCHK(hipEventCreateWithFlags(&ev_look_, hipEventDisableTiming));
@@ -1479,7 +1537,7 @@ class Impl final : public GpuFactor {
     drop_graphs();
     void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
-                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
+                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_c_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
@@ -1627,22 +1685,104 @@ class Impl final : public GpuFactor {
     const int64_t cc = nc_ * nrhs;
     if (!on_device) HIPCHK(hipMemcpyAsync(d_bc_, b, cc * sizeof(double), hipMemcpyHostToDevice, s_));
     double* bc = on_device ? b : d_bc_;
+    const size_t nn = static_cast<size_t>(std::max(nseg_, 1)) * nrhs;
+    std::vector<double> bn(nn), rn(nn);
+    run_solve(bc, nrhs, tol, max_steps, bn.data(), rn.data(), true);
+    if (!on_device) HIPCHK(hipMemcpyAsync(b, d_bc_, cc * sizeof(double), hipMemcpyDeviceToHost, s_));
+    HIPCHK(hipStreamSynchronize(s_));
+  }
+
+  double solve_internal(const double* d_b, double* d_x, int nrhs, double tol, int max_steps) override {
+    if (nrhs <= 0 || n_ == 0) return -1.0;
+    ensure_vectors(nrhs);
+    const int64_t cnt = n_ * nrhs;
+    HIPCHK(hipMemcpyAsync(w_.b, d_b, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    const size_t nn = static_cast<size_t>(std::max(nseg_, 1)) * nrhs;
+    std::vector<double> bn(nn), rn(nn);
+    if (max_steps > 0) {
+      refine(nrhs, tol, max_steps, bn.data(), rn.data());
+    } else {
+      HIPCHK(hipMemcpyAsync(w_.x, w_.b, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
+      solve_device(w_.x, nrhs);
+      tm_.rel_residual = -1.0;
+    }
+    HIPCHK(hipMemcpyAsync(d_x, w_.x, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    HIPCHK(hipStreamSynchronize(s_));
+    return tm_.rel_residual;
+  }
+
+  void set_fallback(SolveFallback f) override { fallback_ = std::move(f); }
+  const double* device_lower_values() const override { return d_lval_; }
+
+  // The solve of solve_device() without any device allocation: every work
+  // vector is carved from the file-scope scratch array of the device that
+  // holds this factor, and the kernels run eagerly (no graph is
+  // instantiated).  Single problems only (no segments).
+  bool solve_noalloc(double* d_b, int nrhs, double tol, int max_steps) override {
+    if (nrhs <= 0 || nc_ == 0) return true;
+    if (nseg_ > 0) throw std::runtime_error("the allocation-free solve serves single-problem handles only");
+    if (nrhs > kNoAllocMaxRhs) {
+      char m[160];
+      snprintf(m, sizeof m, "allocation-free solve: %d right-hand sides, at most %d", nrhs, kNoAllocMaxRhs);
+      throw std::runtime_error(m);
+    }
+    const size_t need = noalloc_elems(nrhs);
+    if (need > kScratchElems) {
+      char m[200];
+      snprintf(m, sizeof m, "allocation-free solve needs %zu bytes of scratch for %d right-hand sides; the maximum is %zu",
+               need * sizeof(double), nrhs, kScratchElems * sizeof(double));
+      throw std::runtime_error(m);
+    }
+    DeviceGuard guard(dev_);
+    double* base = scratch_ptr(dev_);
+    const SolveWork saved = w_;
+    const bool saved_eager = eager_;
+    w_ = carve(base, nrhs);
+    eager_ = true;
+    double bn[kNoAllocMaxRhs], rn[kNoAllocMaxRhs];
+    try {
+      run_solve(d_b, nrhs, tol, max_steps, bn, rn, false);
+      HIPCHK(hipStreamSynchronize(s_));
+    } catch (...) {
+      w_ = saved;
+      eager_ = saved_eager;
+      throw;
+    }
+    w_ = saved;
+    eager_ = saved_eager;
+    return max_steps <= 0 || tm_.rel_residual <= tol;
+  }
+
+  // caller numbering in bc (device) -> internal b, solve (and refine), back
+  // into bc; when the refinement misses tol and a fallback is set (and
+  // allowed), the fallback solves the internal right-hand sides again
+  void run_solve(double* bc, int nrhs, double tol, int max_steps, double* bn, double* rn, bool allow_fallback) {
+    const int64_t cc = nc_ * nrhs;
+    tm_.fell_back = 0;
+    tm_.fallback_rel = -1.0;
     HIPCHK(hipEventRecord(e0_, s_));
     if (n_ > 0) {
-      launch(k_perm_in, (cc + 255) / 256, 256, d_eq_, bc, d_b_, nc_, n_, nrhs);
+      launch(k_perm_in, (cc + 255) / 256, 256, d_eq_, bc, w_.b, nc_, n_, nrhs);
       if (max_steps > 0) {
-        refine(nrhs, tol, max_steps);
+        refine(nrhs, tol, max_steps, bn, rn);
+        if (allow_fallback && fallback_ && !(tm_.rel_residual <= tol)) {
+          HIPCHK(hipStreamSynchronize(s_));
+          const double rel = fallback_(w_.b, w_.x, nrhs);
+          if (rel >= 0.0) {
+            tm_.fell_back = 1;
+            tm_.fallback_rel = rel;
+          }
+        }
       } else {
-        HIPCHK(hipMemcpyAsync(d_x_, d_b_, n_ * nrhs * sizeof(double), hipMemcpyDeviceToDevice, s_));
-        solve_device(d_x_, nrhs);
+        HIPCHK(hipMemcpyAsync(w_.x, w_.b, n_ * nrhs * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        solve_device(w_.x, nrhs);
         tm_.refine_steps = 0;
         tm_.rel_residual = -1.0;
       }
     }
-    launch(k_perm_out, (cc + 255) / 256, 256, d_eq_, d_iso_, d_x_, bc, nc_, n_, nrhs);
+    launch(k_perm_out, (cc + 255) / 256, 256, d_eq_, d_iso_, w_.x, bc, nc_, n_, nrhs);
     HIPCHK(hipEventRecord(e1_, s_));
-    if (!on_device) HIPCHK(hipMemcpyAsync(b, d_bc_, cc * sizeof(double), hipMemcpyDeviceToHost, s_));
-    HIPCHK(hipStreamSynchronize(s_));
+    HIPCHK(hipEventSynchronize(e1_));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
     tm_.solve_ms = ms;
@@ -1786,8 +1926,8 @@ class Impl final : public GpuFactor {
     if (nrhs <= cap_rhs_) return;
     drop_graphs();
     for (void* p : {static_cast<void*>(d_t_), static_cast<void*>(d_x_), static_cast<void*>(d_part_),
-                    static_cast<void*>(d_b_), static_cast<void*>(d_r_), static_cast<void*>(d_red_),
-                    static_cast<void*>(d_nrm_), static_cast<void*>(d_bc_)})
+                    static_cast<void*>(d_b_), static_cast<void*>(d_r_), static_cast<void*>(d_c_),
+                    static_cast<void*>(d_red_), static_cast<void*>(d_nrm_), static_cast<void*>(d_bc_)})
       if (p) HIPCHK(hipFree(p));
     cap_rhs_ = nrhs;
     d_t_ = dalloc<double>(static_cast<size_t>(P_.tvec_elems) * nrhs, bytes_);
@@ -1796,20 +1936,56 @@ class Impl final : public GpuFactor {
     d_x_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_b_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
     d_r_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
-    d_part_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(std::max(P_.max_l21, P_.max_sbpart), 1)) * nrhs * BC,
-                             bytes_);
+    d_c_ = dalloc<double>(static_cast<size_t>(n_) * nrhs, bytes_);
+    d_part_ = dalloc<double>(part_elems() * nrhs, bytes_);
     d_red_ = dalloc<double>(static_cast<size_t>(kRedBlocks) * nrhs, bytes_);
     d_nrm_ = dalloc<double>(nrhs, bytes_);
     d_bc_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(nc_, 1)) * nrhs, bytes_);
+    w_ = SolveWork{d_t_, d_ys_, d_part_, d_x_, d_b_, d_r_, d_c_, d_red_, d_nrm_};
   }
 
-  // x = A^-1 b with FP64 refinement; b in d_b_ (internal numbering), result in d_x_
-  void refine(int nrhs, double tol, int max_steps) {
+  size_t part_elems() const {
+    return static_cast<size_t>(std::max<int64_t>(std::max(P_.max_l21, P_.max_sbpart), 1)) * BC;
+  }
+
+  // the work vectors of one solve of nrhs columns, each start 256-byte aligned
+  static size_t up32(size_t e) { return (e + 31) / 32 * 32; }
+  size_t noalloc_elems(int nrhs) const {
+    const size_t r = static_cast<size_t>(nrhs), tv = static_cast<size_t>(P_.tvec_elems), n = static_cast<size_t>(n_);
+    return 2 * up32(tv * r) + up32(part_elems() * r) + 4 * up32(n * r) + up32(kRedBlocks * r) + up32(r);
+  }
+  SolveWork carve(double* base, int nrhs) const {
+    const size_t r = static_cast<size_t>(nrhs), tv = static_cast<size_t>(P_.tvec_elems), n = static_cast<size_t>(n_);
+    SolveWork w;
+    double* p = base;
+    auto take = [&](size_t e) {
+      double* q = p;
+      p += up32(e);
+      return q;
+    };
+    w.t = take(tv * r);
+    w.ys = take(tv * r);
+    w.part = take(part_elems() * r);
+    w.x = take(n * r);
+    w.b = take(n * r);
+    w.r = take(n * r);
+    w.c = take(n * r);
+    w.red = take(kRedBlocks * r);
+    w.nrm = take(r);
+    return w;
+  }
+
+  // x = A^-1 b with FP64 refinement; b in w_.b (internal numbering), result
+  // in w_.x; bn and rn are host arrays of (segments x nrhs) entries.  The
+  // residuals (and the corrections solved in place of them) alternate
+  // between w_.r and w_.c, so the last correction is still at hand when the
+  // residual it produced is measured: a correction that raised the worst
+  // residual is subtracted again, and the iterate before it is the result.
+  void refine(int nrhs, double tol, int max_steps, double* bn, double* rn) {
     const int64_t cnt = n_ * nrhs;
-    HIPCHK(hipMemcpyAsync(d_x_, d_b_, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
-    solve_device(d_x_, nrhs);
+    HIPCHK(hipMemcpyAsync(w_.x, w_.b, cnt * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    solve_device(w_.x, nrhs);
     const int nseg = nseg_ > 0 ? nseg_ : 1;
-    std::vector<double> bn(static_cast<size_t>(nseg) * nrhs), rn(static_cast<size_t>(nseg) * nrhs);
     auto norms = [&](const double* v, double* out) {
       if (nseg_ > 0) {
         if (!d_segsq_ || segsq_cap_ < nseg * nrhs) {
@@ -1824,13 +2000,16 @@ class Impl final : public GpuFactor {
         sumsq(v, nrhs, out);
       }
     };
-    norms(d_b_, bn.data());
+    norms(w_.b, bn);
     int steps = 0;
     double worst = 0.0, prev = HUGE_VAL;
+    double* last = nullptr;  // the last correction added to x
     for (double& h : tm_.refine_history) h = -1.0;
+    tm_.refine_undone = 0;
     for (;;) {
-      launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, d_x_, d_b_, d_r_, n_, nrhs);
-      norms(d_r_, rn.data());
+      double* r = (steps % 2 == 0) ? w_.r : w_.c;
+      launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      norms(r, rn);
       worst = 0.0;
       bool done = true;
       for (int j = 0; j < nseg * nrhs; ++j) {
@@ -1839,12 +2018,21 @@ class Impl final : public GpuFactor {
         if (!(rel <= tol)) done = false;
       }
       if (steps < 4) tm_.refine_history[steps] = worst;
+      if (last && !(worst <= prev)) {
+        // the last correction raised the worst residual (or made it NaN)
+        launch(k_axpy, (cnt + 255) / 256, 256, w_.x, last, cnt, -1.0);
+        worst = prev;
+        --steps;
+        tm_.refine_undone = 1;
+        break;
+      }
       // stop at the target, at the step limit, or when a correction no longer
       // halves the worst residual (the FP64 rounding floor of the residual)
       if (done || steps >= max_steps || !(worst < 0.5 * prev)) break;
       prev = worst;
-      solve_device(d_r_, nrhs);
-      launch(k_axpy, (cnt + 255) / 256, 256, d_x_, d_r_, cnt);
+      solve_device(r, nrhs);
+      launch(k_axpy, (cnt + 255) / 256, 256, w_.x, r, cnt, 1.0);
+      last = r;
       ++steps;
     }
     tm_.refine_steps = steps;
@@ -1860,9 +2048,9 @@ class Impl final : public GpuFactor {
   }
 
   void sumsq(const double* v, int nrhs, double* host_out) {
-    launch(k_sumsq_partial, kRedBlocks, 256, v, n_, nrhs, d_red_);
-    launch(k_sumsq_final, nrhs, 64, d_red_, kRedBlocks, nrhs, d_nrm_);
-    HIPCHK(hipMemcpyAsync(host_out, d_nrm_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+    launch(k_sumsq_partial, kRedBlocks, 256, v, n_, nrhs, w_.red);
+    launch(k_sumsq_final, nrhs, 64, w_.red, kRedBlocks, nrhs, w_.nrm);
+    HIPCHK(hipMemcpyAsync(host_out, w_.nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIPCHK(hipStreamSynchronize(s_));
   }
 
@@ -1883,7 +2071,7 @@ class Impl final : public GpuFactor {
   // The solve is a fixed sequence of kernel launches on fixed buffers: it is
   // captured once per (vector, nrhs) into a HIP graph and replayed.
   void solve_device(double* dx, int nrhs) {
-    if (!use_graphs_) {
+    if (!use_graphs_ || eager_) {
       solve_eager(dx, nrhs);
       return;
     }
@@ -1925,31 +2113,31 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, w_.t, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, w_.t, tv,
                    nrhs);
       }
       if (!lp.solve.empty())
-        launch(k_fwd_first<R>, lp.solve[0].diag.count, 256, d_fd_, d_diag_, lp.solve[0].diag.begin, d_vbuf_, d_t_,
+        launch(k_fwd_first<R>, lp.solve[0].diag.count, 256, d_fd_, d_diag_, lp.solve[0].diag.begin, d_vbuf_, w_.t,
                tv, nrhs);
       for (const SolveStep& sp : lp.solve)
-        launch(k_fwd_step<T, R>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_vbuf_, d_t_, tv,
+        launch(k_fwd_step<T, R>, sp.rows.count, 256, d_fd_, d_srow_ + sp.rows.begin, d_arena_, d_vbuf_, w_.t, tv,
                nrhs);
-      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const BwdPlan& bp = lp.bwd;
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
-      launch(k_l21_partial<T, R>, bp.l21.count, 256, d_fd_, d_l21_ + bp.l21.begin, d_arena_, d_t_, tv, nrhs, d_part_);
-      launch(k_l21_combine, bp.comb.count, 256, d_fd_, d_l21c_ + bp.comb.begin, d_part_, d_t_, tv, nrhs);
-      launch(k_bwd_last<R>, bp.last.count, 256, d_fd_, d_diag_, d_last_ + bp.last.begin, d_vbuf_, d_t_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
+      launch(k_l21_partial<T, R>, bp.l21.count, 256, d_fd_, d_l21_ + bp.l21.begin, d_arena_, w_.t, tv, nrhs, w_.part);
+      launch(k_l21_combine, bp.comb.count, 256, d_fd_, d_l21c_ + bp.comb.begin, w_.part, w_.t, tv, nrhs);
+      launch(k_bwd_last<R>, bp.last.count, 256, d_fd_, d_diag_, d_last_ + bp.last.begin, d_vbuf_, w_.t, tv, nrhs);
       for (const Range& st : bp.steps)
-        launch(k_bwd_step<T, R>, st.count, 256, d_fd_, d_btile_ + st.begin, d_arena_, d_vbuf_, d_t_, tv, nrhs);
-      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+        launch(k_bwd_step<T, R>, st.count, 256, d_fd_, d_btile_ + st.begin, d_arena_, d_vbuf_, w_.t, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, dx, n_, tv, nrhs);
     }
   }
 
@@ -1961,48 +2149,48 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, w_.t, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, w_.t, tv,
                    nrhs);
       }
       for (const SbFwdStep& st : lp.sfwd) {
         if (st.chain.count > 0) {
           hipLaunchKernelGGL(k_sfwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, d_t_, d_ys_, tv, nrhs);
+                             d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, w_.t, w_.ys, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
         if (st.rows.count > 0) {
           auto* kr = &k_sfwd_rows<T, R>;
           hipLaunchKernelGGL(kr, dim3(grid(st.rows.count), nch), dim3(256), 0, s_, d_fd_, d_sbrow_ + st.rows.begin,
-                             d_arena_, d_ys_, d_t_, tv, nrhs);
+                             d_arena_, w_.ys, w_.t, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
       }
-      launch_vec(k_fwd_finish_ys, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_ys_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish_ys, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.ys, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
       for (const SbBwdStep& st : lp.sbwd) {
         if (st.part.count > 0) {
           auto* kp = &k_sbwd_part<T, R>;
           hipLaunchKernelGGL(kp, dim3(grid(st.part.count), nch), dim3(256), 0, s_, d_fd_, d_sbpart_ + st.part.begin,
-                             d_arena_, d_t_, tv, nrhs, d_part_);
+                             d_arena_, w_.t, tv, nrhs, w_.part);
           HIPCHK(hipGetLastError());
         }
         if (st.chain.count > 0) {
           hipLaunchKernelGGL(k_sbwd_comb<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, d_part_, d_t_, d_ys_, tv, nrhs);
+                             d_fd_, d_sbt_ + st.chain.begin, w_.part, w_.t, w_.ys, tv, nrhs);
           HIPCHK(hipGetLastError());
           hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, d_ys_, d_t_, tv, nrhs);
+                             d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, w_.ys, w_.t, tv, nrhs);
           HIPCHK(hipGetLastError());
         }
       }
-      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, dx, n_, tv, nrhs);
     }
   }
 
@@ -2016,30 +2204,30 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, w_.t, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
-        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, d_t_, tv,
+        launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, w_.t, tv,
                    nrhs);
       }
       for (const SbFwdStep& st : lp.sfwd) {
         launch(k_sfwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
-               d_t_, tv, nrhs);
-        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, d_t_, d_t_, tv, nrhs);
+               w_.t, tv, nrhs);
+        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.t, w_.t, tv, nrhs);
       }
-      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, d_dpiv_, dx, n_, tv, nrhs);
+      launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, d_t_, n_, tv, nrhs);
+      launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
       for (const SbBwdStep& st : lp.sbwd) {
-        launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, d_t_, tv, nrhs,
-               d_part_);
+        launch(k_sbwd_part<T, R>, st.part.count, 256, d_fd_, d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs,
+               w_.part);
         launch(k_sbwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
-               d_part_, d_t_, tv, nrhs);
+               w_.part, w_.t, tv, nrhs);
       }
-      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, d_t_, dx, n_, tv, nrhs);
+      launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, dx, n_, tv, nrhs);
     }
   }
 
@@ -2113,8 +2301,14 @@ class Impl final : public GpuFactor {
   int32_t* d_ci_ = nullptr;
   int64_t* d_gmap_ = nullptr;
   double* d_av_ = nullptr;
-  double *d_t_ = nullptr, *d_x_ = nullptr, *d_part_ = nullptr, *d_b_ = nullptr, *d_r_ = nullptr, *d_red_ = nullptr,
-         *d_nrm_ = nullptr;
+  double *d_t_ = nullptr, *d_x_ = nullptr, *d_part_ = nullptr, *d_b_ = nullptr, *d_r_ = nullptr, *d_c_ = nullptr,
+         *d_red_ = nullptr, *d_nrm_ = nullptr;
+  // the work vectors the solve kernels use: the buffers above, or a carve of
+  // the scratch array during solve_noalloc()
+  SolveWork w_;
+  bool eager_ = false;  // solve_device() without graphs (set by solve_noalloc)
+  SolveFallback fallback_;  // solves again when the refinement misses its target
+  int dev_ = 0;         // the device that holds this factor
   double* d_bc_ = nullptr;
   int64_t* d_eq_ = nullptr;
   double* d_iso_ = nullptr;
@@ -2122,6 +2316,14 @@ class Impl final : public GpuFactor {
 };
 
 }  // namespace
+
+void copy_to_host(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
+void copy_to_device(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
 
 std::unique_ptr<GpuFactor> make_gpu_factor(Plan plan, const std::vector<int64_t>& colptr,
                                            const std::vector<int64_t>& rowidx, Precision prec) {

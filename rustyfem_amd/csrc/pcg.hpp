@@ -9,6 +9,8 @@
 
 namespace rf {
 
+struct AmgHost;
+
 struct PcgOptions {
   int krylov = 0;        // 0: PCG (csr.rs:341-380); 1: flexible CG (FCG(1), Notay)
   int vcycle_fp32 = 0;   // V-cycle arithmetic: 0 FP64, 1 FP32 (matrices and vectors)
@@ -16,6 +18,8 @@ struct PcgOptions {
   int degree = 2;        // Chebyshev degree, pre and post
   double cheb_ratio = 20.0;  // Chebyshev interval [rho / ratio, rho]
   int batch = 8;         // iterations between convergence checks on the host
+  bool keep_hierarchy = false;       // keep the host AMG hierarchy (PcgGpu::hierarchy)
+  bool force_amg_breakdown = false;  // test hook: the AMG setup reports a breakdown
 };
 
 struct PcgResult {
@@ -52,11 +56,15 @@ class PcgGpu {
   // the tolerance makes the solve start again with plain AMG-PCG (Chebyshev
   // smoothing, standard PCG), whose result is returned.
   virtual void set_learned_smoother(const double* params, int sweeps) = 0;
+  // the host AMG hierarchy (aggregates, tentative and smoothed prolongators,
+  // level matrices), when the options asked to keep it; else nullptr
+  virtual const AmgHost* hierarchy() const { return nullptr; }
 };
 
 // kind: 0 AMG (smoothed aggregation, rigid body modes), 1 IC(0), 2 Jacobi.  coords /
 // ndof describe the node blocks (natural numbering: node v owns ndof[v]
-// consecutive equations).  Throws when the AMG setup breaks down.
+// consecutive equations).  Throws AmgBreakdown (amg.hpp) when the AMG setup
+// breaks down (src/solver/iterative/amg.rs:289-291, 328-330: Err(())).
 std::unique_ptr<PcgGpu> make_pcg_gpu(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
                                      int kind, int64_t nn, const double* coords, const int64_t* ndof,
                                      const PcgOptions& opt);

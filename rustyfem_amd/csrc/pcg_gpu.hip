@@ -730,6 +730,7 @@ class PcgImpl final : public PcgGpu {
     for (size_t i = 0; i < len && i < v.size(); ++i) out[i] = v[i];
   }
   std::vector<int64_t> level_sizes() const override { return sizes_; }
+  const AmgHost* hierarchy() const override { return kept_.get(); }
   void set_learned_smoother(const double* params, int sweeps) override;
   void apply(const double* r, double* z, bool on_device) override;
 
@@ -768,6 +769,7 @@ class PcgImpl final : public PcgGpu {
   std::vector<int> blk_;   // DOFs per block of each level (finest first, coarsest last)
   double complexity_ = 1.0, setup_ms_ = 0, solve_ms_ = 0, spmv_ms_ = 0;
   int64_t last_it_ = 0;
+  std::unique_ptr<AmgHost> kept_;  // host hierarchy, when the options keep it
 };
 
 double PcgImpl::power_radius(const DevMat& M, const std::vector<double>& diag) {
@@ -870,7 +872,8 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
       DevMat tmp = make_mat(m, 1, 1, true, false);
       return power_radius(tmp, diag);
     };
-    AmgHost H = amg_setup(std::move(a), std::move(bptr), std::move(bm), nb, radius);
+    AmgHost H = amg_setup(std::move(a), std::move(bptr), std::move(bm), nb, radius, opt_.keep_hierarchy);
+    if (opt_.force_amg_breakdown) throw AmgBreakdown("AMG setup: breakdown forced by the caller (test hook)");
     sizes_ = H.sizes;
     complexity_ = H.complexity;
     // coarser level matrices in the V-cycle precision
@@ -884,6 +887,7 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
       coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16()));
     if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
     else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
+    if (opt_.keep_hierarchy) kept_ = std::make_unique<AmgHost>(std::move(H));
   }
   x_.alloc(n * 8);
   r_.alloc(n * 8);

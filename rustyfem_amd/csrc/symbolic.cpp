@@ -1,10 +1,13 @@
 // Nested-dissection symbolic analysis (restates src/sparse.rs:78-275).
 //
-// Coordinate ties inside a sort are broken by the current position (stable
-// sort).  The reference uses Rust's unstable sort there (src/sparse.rs:108);
-// its tie order decides only the labels inside a supernode, never the tree,
-// the supernode sizes or the front row sets.
+// Node sets are sorted by a coordinate with the restated unstable sort of
+// Rust's standard library (rust_sort.hpp), as src/sparse.rs:108 sorts them:
+// its order among equal coordinates decides the labels inside a supernode.
+// The sorts by first equation (src/sparse.rs:224, 260, 263) have distinct
+// keys; they use the same sort.
 #include "symbolic.hpp"
+
+#include "rust_sort.hpp"
 
 #include <algorithm>
 #include <array>
@@ -41,7 +44,7 @@ class Dissector {
     double best_score = 0.0, best_t = 0.0;
     for (int ax : axes) {
       if (!(hi[ax] > lo[ax])) continue;
-      std::stable_sort(set.begin(), set.end(), [&](int64_t a, int64_t b) { return x_[3 * a + ax] < x_[3 * b + ax]; });
+      rust_sort_unstable(set.data(), set.size(), [&](int64_t a, int64_t b) { return x_[3 * a + ax] < x_[3 * b + ax]; });
       for (size_t q : {n / 2, 2 * n / 5, 3 * n / 5}) {
         const double t = x_[3 * set[q] + ax];
         const size_t nl = std::partition_point(set.begin(), set.end(), [&](int64_t v) { return x_[3 * v + ax] < t; }) -
@@ -160,7 +163,7 @@ Symbolic analyse(const double* coords, int64_t nn, const std::vector<int64_t>& a
       for (int64_t p = adj_ptr[v]; p < adj_ptr[v + 1]; ++p) take(adj_idx[p]);
     for (int64_t c : T[t].children)
       for (int64_t w : T[c].row_nodes) take(w);
-    std::stable_sort(list.begin(), list.end(), [&](int64_t a, int64_t b) { return s.first_dof[a] < s.first_dof[b]; });
+    rust_sort_unstable(list.data(), list.size(), [&](int64_t a, int64_t b) { return s.first_dof[a] < s.first_dof[b]; });
     for (int64_t w : list)
       for (int64_t d = 0; d < ndof[w]; ++d) T[t].rows.push_back(s.first_dof[w] + d);
     const double k = static_cast<double>(T[t].k), u = static_cast<double>(T[t].rows.size());
@@ -191,14 +194,14 @@ void Symbolic::pattern(const std::vector<int64_t>& adj_ptr, const std::vector<in
   const int64_t nn = static_cast<int64_t>(first_dof.size());
   std::vector<int64_t> order(nn);
   for (int64_t i = 0; i < nn; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return first_dof[a] < first_dof[b]; });
+  rust_sort_unstable(order.data(), order.size(), [&](int64_t a, int64_t b) { return first_dof[a] < first_dof[b]; });
   colptr.assign(n + 1, 0);
   rowidx.clear();
   std::vector<int64_t> nb;
   for (int64_t v : order) {
     nb.assign(adj_idx.begin() + adj_ptr[v], adj_idx.begin() + adj_ptr[v + 1]);
     nb.push_back(v);
-    std::stable_sort(nb.begin(), nb.end(), [&](int64_t a, int64_t b) { return first_dof[a] < first_dof[b]; });
+    rust_sort_unstable(nb.data(), nb.size(), [&](int64_t a, int64_t b) { return first_dof[a] < first_dof[b]; });
     const int64_t f = first_dof[v];
     for (int64_t c = f; c < f + ndof[v]; ++c) {
       for (int64_t w : nb) {

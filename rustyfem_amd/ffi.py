@@ -25,7 +25,8 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_factor_batch", "rustyfem_refactor_batch", "rustyfem_solve_batch", "rustyfem_solve_batch_device",
            "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch",
            "rustyfem_pcg_set_learned_smoother", "rustyfem_mpnn_nparams", "rustyfem_mpnn_create", "rustyfem_mpnn_loss",
-           "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition", "rustyfem_chebyshev"]
+           "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition", "rustyfem_chebyshev",
+           "rustyfem_pcg_setup_ex", "rustyfem_pcg_level", "rustyfem_amg_aggregate"]
 
 _lib = None
 
@@ -33,7 +34,12 @@ _lib = None
 class PcgOptions(C.Structure):
     """rustyfem_pcg_options (include/rustyfem.h)."""
     _fields_ = [("krylov", C.c_int32), ("vcycle_fp32", C.c_int32), ("smoother", C.c_int32), ("degree", C.c_int32),
-                ("cheb_ratio", C.c_double)]
+                ("cheb_ratio", C.c_double), ("flags", C.c_int32), ("reserved", C.c_int32)]
+
+
+PCG_KEEP_HIERARCHY = 1
+PCG_NO_DIRECT_FALLBACK = 2
+PCG_FORCE_AMG_BREAKDOWN = 4
 
 
 class Options(C.Structure):
@@ -87,6 +93,12 @@ def lib():
         L.rustyfem_factor_stats.argtypes = [vp, vp]
         L.rustyfem_pcg_setup.restype = vp
         L.rustyfem_pcg_setup.argtypes = [sz, vp, vp, vp, C.c_int32, sz, vp, vp, C.POINTER(PcgOptions)]
+        L.rustyfem_pcg_setup_ex.restype = vp
+        L.rustyfem_pcg_setup_ex.argtypes = [sz, vp, vp, vp, C.c_int32, sz, vp, vp, C.POINTER(PcgOptions), i64p]
+        L.rustyfem_pcg_level.restype = C.c_int32
+        L.rustyfem_pcg_level.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp]
+        L.rustyfem_amg_aggregate.restype = C.c_int64
+        L.rustyfem_amg_aggregate.argtypes = [sz, vp, vp, vp, sz, vp, C.c_double, vp]
         L.rustyfem_pcg_solve.restype = C.c_int64
         L.rustyfem_pcg_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int64, C.c_int32, f64p, C.POINTER(C.c_int32)]
         L.rustyfem_pcg_stats.restype = None
@@ -173,12 +185,15 @@ _STATS_KEYS = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residu
 
 
 def _stats_dict(fn, h):
-    """rustyfem_stats / rustyfem_batch_stats as a dict (21 entries, rustyfem.h)."""
-    o = np.zeros(21)
-    fn(h, o.ctypes.data, 21)
+    """rustyfem_stats / rustyfem_batch_stats as a dict (24 entries, rustyfem.h)."""
+    o = np.zeros(24)
+    fn(h, o.ctypes.data, 24)
     d = dict(zip(_STATS_KEYS, o.tolist()))
     d["refine_history"] = [float(v) for v in o[16:20] if v >= 0]
     d["ilv_launches"] = float(o[20])
+    d["refine_undone"] = bool(o[21])
+    d["fp64_fallback"] = bool(o[22])
+    d["fallback_rel_residual"] = float(o[23])
     return d
 
 
@@ -299,7 +314,7 @@ class Pcg:
     numbering (rustyfem_pcg_setup / _solve)."""
 
     def __init__(self, n, colptr, rowidx, val, kind="amg", coords=None, ndof=None, krylov="pcg", vcycle="fp64",
-                 smoother="chebyshev", degree=2, cheb_ratio=20.0):
+                 smoother="chebyshev", degree=2, cheb_ratio=20.0, flags=0):
         self.n = int(n)
         self._keep = [np.ascontiguousarray(colptr, dtype=np.int64), np.ascontiguousarray(rowidx, dtype=np.int64),
                       np.ascontiguousarray(val, dtype=np.float64)]
@@ -310,10 +325,12 @@ class Pcg:
         nd = np.ascontiguousarray(ndof, dtype=np.int64)
         self._keep += [c, nd]
         o = PcgOptions(1 if krylov == "fcg" else 0, 1 if vcycle == "fp32" else 0, 1 if smoother == "jacobi" else 0,
-                       int(degree), float(cheb_ratio))
-        self._h = lib().rustyfem_pcg_setup(self.n, self._keep[0].ctypes.data, self._keep[1].ctypes.data,
-                                           self._keep[2].ctypes.data, KIND[kind], c.shape[0], c.ctypes.data,
-                                           nd.ctypes.data, C.byref(o))
+                       int(degree), float(cheb_ratio), int(flags), 0)
+        err = C.c_int64(-1)
+        self._h = lib().rustyfem_pcg_setup_ex(self.n, self._keep[0].ctypes.data, self._keep[1].ctypes.data,
+                                              self._keep[2].ctypes.data, KIND[kind], c.shape[0], c.ctypes.data,
+                                              nd.ctypes.data, C.byref(o), C.byref(err))
+        self.err_eq = int(err.value)
         if not self._h:
             raise RuntimeError(last_error())
 
@@ -336,16 +353,50 @@ class Pcg:
             raise RuntimeError(last_error())
         return int(it), float(rel.value), bool(conv.value)
 
+    def level(self, l):
+        """Host copy of AMG level l (RUSTYFEM_PCG_KEEP_HIERARCHY): A, P~, P as
+        scipy CSR matrices, the aggregates, block boundaries and near-null space
+        B; the coarsest level carries only A, bptr and B."""
+        import scipy.sparse as sp
+        info = np.zeros(8, dtype=np.int64)
+        if lib().rustyfem_pcg_level(self._h, int(l), -1, info.ctypes.data, None, None) != 0:
+            return None
+        rows, nnz_a, nblk, nagg, nbm, ncols, nnz_pt, nnz_p = (int(v) for v in info)
+
+        def csr(what, nr, nc, nnz):
+            ptr = np.zeros(nr + 1, dtype=np.int64)
+            idx = np.zeros(max(nnz, 1), dtype=np.int32)
+            val = np.zeros(max(nnz, 1))
+            if lib().rustyfem_pcg_level(self._h, int(l), what, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data):
+                return None
+            return sp.csr_matrix((val[:nnz], idx[:nnz], ptr), shape=(nr, nc))
+
+        out = {"A": csr(0, rows, rows, nnz_a), "nblk": nblk, "nagg": nagg, "nbm": nbm}
+        bptr = np.zeros(nblk + 1, dtype=np.int64)
+        lib().rustyfem_pcg_level(self._h, int(l), 4, bptr.ctypes.data, None, None)
+        out["bptr"] = bptr
+        B = np.zeros(max(rows * nbm, 1))
+        lib().rustyfem_pcg_level(self._h, int(l), 5, None, None, B.ctypes.data)
+        out["B"] = B[:rows * nbm].reshape(rows, nbm)
+        if ncols > 0:
+            out["Pt"] = csr(1, rows, ncols, nnz_pt)
+            out["P"] = csr(2, rows, ncols, nnz_p)
+            agg = np.zeros(nblk, dtype=np.int64)
+            lib().rustyfem_pcg_level(self._h, int(l), 3, agg.ctypes.data, None, None)
+            out["agg"] = agg
+        return out
+
     def stats(self):
-        o = np.zeros(40)
-        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 40)
+        o = np.zeros(43)
+        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 43)
         lv = int(o[2])
         return {"setup_ms": o[0], "solve_ms": o[1], "levels": lv, "complexity": o[3], "spmv_ms": o[4],
                 "spmv_bytes": o[5], "coarse": int(o[6]), "iterations": int(o[7]),
                 "sizes": [int(v) for v in o[8:8 + min(lv, 12)]], "ic0_shift": o[20],
                 "ic0_forward_levels": int(o[21]), "ic0_backward_levels": int(o[22]), "ic0_attempts": int(o[23]),
                 "fallback": bool(o[24]), "true_rel_residual": o[25], "abandoned_iterations": int(o[26]),
-                "learned_sweeps": int(o[27]), "blocks": [int(b) for b in o[28:28 + min(lv, 12)]]}
+                "learned_sweeps": int(o[27]), "blocks": [int(b) for b in o[28:28 + min(lv, 12)]],
+                "direct_reason": int(o[40]), "last_direct": bool(o[41]), "direct_rel_residual": o[42]}
 
     def precondition(self, r):
         """z = M r, one application of the preconditioner."""
@@ -498,3 +549,21 @@ class Batch:
         if getattr(self, "_h", None):
             lib().rustyfem_free_batch(self._h)
             self._h = None
+
+
+def amg_aggregate(a, bptr, theta):
+    """Vanek's three passes of one level (rustyfem_amg_aggregate) on a full
+    symmetric scipy matrix with DOF blocks bptr: (aggregate per block, count)."""
+    import scipy.sparse as sp
+    m = sp.csr_matrix(a)
+    m.sort_indices()
+    ptr = np.ascontiguousarray(m.indptr, dtype=np.int64)
+    idx = np.ascontiguousarray(m.indices, dtype=np.int32)
+    val = np.ascontiguousarray(m.data, dtype=np.float64)
+    bp = np.ascontiguousarray(bptr, dtype=np.int64)
+    agg = np.zeros(bp.size - 1, dtype=np.int64)
+    n = lib().rustyfem_amg_aggregate(m.shape[0], ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, bp.size - 1,
+                                     bp.ctypes.data, float(theta), agg.ctypes.data)
+    if n < 0:
+        raise RuntimeError(last_error())
+    return agg, int(n)
