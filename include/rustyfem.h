@@ -92,6 +92,21 @@ int32_t rustyfem_refactor(Handle *h, const double *data, int64_t *err_dof);
  * place); runs refinement when the handle's options ask for it (additive). */
 int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
 
+/* rustyfem_solve_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The solve's work vectors come from one fixed scratch array per device
+ * (64 MiB, held until the process ends); the kernels run eagerly (no graph
+ * is instantiated).  Limits: positive definite single-problem handles, at
+ * most 64 right-hand sides, and work vectors (2 sum(m) + 4 n + the partial
+ * sums, in doubles, per right-hand side) that fit the 64 MiB; larger calls
+ * are rejected with an error (-1, rustyfem_last_error).  Calls that overlap
+ * on one device race on its scratch array (calls on one handle are
+ * serialized).  The refinement runs as in rustyfem_solve_device, but a mixed
+ * handle that misses its refinement target is not repeated in FP64 (that
+ * would allocate): the call returns 1 instead.  0 on success. */
+int32_t rustyfem_solve_device_noalloc(const Handle *h, size_t nrhs, double *d_b);
+
 /* Statistics of the last factorization / solve (additive):
  * out[0] factor ms, [1] solve ms, [2] refined-solve ms, [3] refinement steps,
  * [4] relative residual after refinement (-1 if not refined), [5] flops,

@@ -640,6 +640,22 @@ int32_t rustyfem_solve_device(const Handle* h, size_t nrhs, double* d_b) {
   return -1;
 }
 
+int32_t rustyfem_solve_device_noalloc(const Handle* h, size_t nrhs, double* d_b) {
+  if (h->n == 0 || nrhs == 0) return 0;
+  try {
+    if (h->mode != Mode::Pd) throw std::runtime_error("device solves need a positive definite handle");
+    if (nrhs > static_cast<size_t>(INT32_MAX)) throw std::runtime_error("too many right-hand sides");
+    std::lock_guard<std::mutex> lock(h->solve_mu);
+    const bool met = h->gpu->solve_noalloc(d_b, static_cast<int>(nrhs), h->opt.refine_tol, h->opt.max_refine);
+    return met ? 0 : 1;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
 void rustyfem_stats(const Handle* h, double* out, size_t len) {
   double v[NSTATS] = {0};
   if (h->gpu) {

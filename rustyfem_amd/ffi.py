@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "librustyfem_ffi.so")
 
 EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", "rustyfem_inertia",
            "rustyfem_solve", "rustyfem_info", "rustyfem_order", "rustyfem_free", "rustyfem_factor_ex",
-           "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
+           "rustyfem_refactor", "rustyfem_solve_device", "rustyfem_solve_device_noalloc", "rustyfem_stats", "rustyfem_last_error", "rustyfem_analyse",
            "rustyfem_check_plan", "rustyfem_pivots", "rustyfem_factor_stats", "rustyfem_pcg_setup",
            "rustyfem_pcg_solve", "rustyfem_pcg_stats", "rustyfem_pcg_free", "rustyfem_solve_lower",
            "rustyfem_factor_batch", "rustyfem_refactor_batch", "rustyfem_solve_batch", "rustyfem_solve_batch_device",
@@ -71,6 +71,8 @@ def lib():
         L.rustyfem_solve.argtypes = [vp, sz, vp]
         L.rustyfem_solve_device.restype = C.c_int32
         L.rustyfem_solve_device.argtypes = [vp, sz, vp]
+        L.rustyfem_solve_device_noalloc.restype = C.c_int32
+        L.rustyfem_solve_device_noalloc.argtypes = [vp, sz, vp]
         L.rustyfem_info.restype = None
         L.rustyfem_info.argtypes = [vp, vp]
         L.rustyfem_order.restype = None
@@ -257,6 +259,14 @@ class Handle:
         rc = lib().rustyfem_solve_device(self._h, int(nrhs), ptr)
         if rc != 0:
             raise RuntimeError("rustyfem_solve_device failed: " + last_error())
+
+    def solve_device_noalloc(self, ptr, nrhs):
+        """rustyfem_solve_device_noalloc; returns True when the refinement met
+        its target (always True without refinement)."""
+        rc = lib().rustyfem_solve_device_noalloc(self._h, int(nrhs), ptr)
+        if rc < 0:
+            raise RuntimeError("rustyfem_solve_device_noalloc failed: " + last_error())
+        return rc == 0
 
     def refactor(self, data=None):
         err = C.c_int64(0)
