@@ -125,8 +125,9 @@ int32_t rustyfem_solve_device_noalloc(const Handle *h, size_t nrhs, double *d_b)
  * solve of a mixed-precision handle missed the refinement target and was
  * repeated with an FP64 factorization of the same values (built at the first
  * such solve, kept until the next refactorization), [23] the relative
- * residual that repeat reached (-1 if none ran).
- * At most 24 entries are written. */
+ * residual that repeat reached (-1 if none ran), [24] how many GEMM launches
+ * the autotuning moved to the kernel with 16-byte accesses (FP32 only).
+ * At most 25 entries are written. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 
 /* Last error message of the calling thread ("" if none) (additive). */

@@ -382,7 +382,7 @@ bool batch_numeric(BatchHandle* h, const double* const* data, int64_t* err) {
 
 // entries of rustyfem_stats / rustyfem_batch_stats that come from the
 // factorization's timings (the layout is documented in rustyfem.h)
-constexpr size_t NSTATS = 24;
+constexpr size_t NSTATS = 25;
 void fill_timings(const rf::Timings& t, double* v) {
   v[0] = t.factor_ms;
   v[1] = t.solve_ms;
@@ -400,6 +400,7 @@ void fill_timings(const rf::Timings& t, double* v) {
   v[21] = t.refine_undone;
   v[22] = t.fell_back;
   v[23] = t.fallback_rel;
+  v[24] = t.q_launches;
 }
 
 }  // namespace

@@ -7,6 +7,15 @@
 
 namespace rf {
 
+// Workgroup bid of a launch runs on XCD bid % 8 (round-robin dispatch).  The
+// tile of bid is chosen so that every XCD works through a contiguous run of
+// the tile list (tiles of one front column block share their B panel, which
+// then stays in that XCD's L2).  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int x = bid & 7, s = bid >> 3, q = nb >> 3, r = nb & 7;
+  return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
 // C[r0:r0+128, c0:min(c0+128, cend)) -= (L[r0:, k0:k1] D[k0:k1]) L[c0:, k0:k1]^T
 // on one 128 x 128 tile of a column-major front.
 //
@@ -269,6 +278,144 @@ __device__ __forceinline__ void gemm_tile_ilv(const FrontDesc* __restrict__ fd, 
         if (col < t.cend) F[row + static_cast<int64_t>(col) * m] = cv[bj][r] - acc[bi][bj][r];
       }
   }
+}
+
+// Four floats at a 4-byte aligned address: one 16-byte access (gfx950 runs
+// in unaligned access mode; the front leading dimension m is arbitrary).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// FP32 Schur-update tile with 16-byte accesses: the same 128 x 128 tile and
+// the same MFMA sequence per accumulator as gemm_tile<float> (k = 16 h + 4 g
+// + s at step s of chunk h for lane group g, k increasing), so the results
+// are identical; what differs is the data movement.
+//  - Global loads: thread (rq, kq) loads rows 4 rq .. 4 rq + 3 of the columns
+//    4 kq .. 4 kq + 3 of both panels, four 16-byte loads per panel (a wave
+//    reads two whole 512-byte column segments per load).  A quad of rows
+//    that starts at or beyond the last valid row starts at that row instead;
+//    it then reads up to three entries past the column (the next column's
+//    unused upper part, or the arena's padding), which reach only C rows
+//    that are not stored.  Columns beyond k1 are clamped to k1 - 1 with D = 0.
+//  - LDS: [k quad][row][k % 4], two buffers, one barrier per K tile: the
+//    next tile's loads are in flight during the current tile's products and
+//    land in the other buffer.  D multiplies the A panel when it is stored.
+//    A lane's MFMA operands for four steps are one 16-byte LDS read.
+//  - C: the A panel is the MFMA's first operand, so output register r of
+//    lane l is row 4 (l / 16) + r, column l % 16 of the 16 x 16 block: every
+//    lane reads and writes four consecutive rows of one column (16 bytes);
+//    a quad that crosses the front's last row is stored entry by entry.
+//  - Tile order: xcd_tile.
+__device__ __forceinline__ void gemm_tile_q(const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                            float* __restrict__ arena, const double* __restrict__ dpiv) {
+  using M = Mfma<float>;
+  using V4 = M::acc_t;
+  constexpr int KT = M::KT;      // 32 k per tile
+  constexpr int NKQ = KT / 4;    // k quads per tile
+  __shared__ __attribute__((aligned(16))) float S[2][2][NKQ][GT_M][4];  // [buffer][A, B][k quad][row][k % 4]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int rq = tid & 31, kq = tid >> 5;
+  const int g = lane >> 4, l16 = lane & 15;
+  const GemmTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const FrontDesc d = fd[t.front];
+  float* F = arena + d.off;
+  const int64_t m = d.m;
+  const int arow = min(t.r0 + 4 * rq, d.m - 1);
+  const int brow = min(t.c0 + 4 * rq, t.cend - 1);
+  // the K tile in flight: its 16-byte row quads of both panels and the D of
+  // its columns (loaded unconditionally, D of the clamped column, so that no
+  // branch waits for the loads before the products)
+  f4u ra[4], rb[4];
+  double rd[4];
+  int kload = 0;
+  auto gload = [&](int kb) {
+    kload = kb;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = min(kb + 4 * kq + j, t.k1 - 1);
+      const float* col = F + static_cast<int64_t>(c) * m;
+      ra[j] = *reinterpret_cast<const f4u*>(col + arow);
+      rb[j] = *reinterpret_cast<const f4u*>(col + brow);
+      rd[j] = dpiv[d.start + c];
+    }
+  };
+  auto store = [&](int buf) {
+    float dk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dk[j] = kload + 4 * kq + j < t.k1 ? static_cast<float>(rd[j]) : 0.0f;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      *reinterpret_cast<v4f*>(&S[buf][0][kq][4 * rq + rr][0]) =
+          v4f{ra[0][rr] * dk[0], ra[1][rr] * dk[1], ra[2][rr] * dk[2], ra[3][rr] * dk[3]};
+      *reinterpret_cast<v4f*>(&S[buf][1][kq][4 * rq + rr][0]) = v4f{rb[0][rr], rb[1][rr], rb[2][rr], rb[3][rr]};
+    }
+  };
+  V4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = V4{0, 0, 0, 0};
+  auto products = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < KT / 16; ++h) {
+      const int q = 4 * h + g;
+      v4f av[4], bv[4];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) av[bi] = *reinterpret_cast<const v4f*>(&S[buf][0][q][wm * 64 + bi * 16 + l16][0]);
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) bv[bj] = *reinterpret_cast<const v4f*>(&S[buf][1][q][wn * 64 + bj * 16 + l16][0]);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(av[bi][s], bv[bj][s], acc[bi][bj]);
+    }
+  };
+  gload(t.k0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int kb = t.k0; kb < t.k1; kb += KT) {
+    const bool more = kb + KT < t.k1;
+    if (more) gload(kb + KT);
+    products(buf);
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int row = t.r0 + wm * 64 + bi * 16 + 4 * g;
+    const int rowc = min(row, d.m - 1);
+    f4u cv[4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      const int colc = min(t.c0 + wn * 64 + bj * 16 + l16, t.cend - 1);
+      cv[bj] = *reinterpret_cast<const f4u*>(F + rowc + static_cast<int64_t>(colc) * m);
+    }
+    if (row >= d.m) continue;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      const int col = t.c0 + wn * 64 + bj * 16 + l16;
+      if (col >= t.cend) continue;
+      const f4u o = cv[bj] - f4u{acc[bi][bj][0], acc[bi][bj][1], acc[bi][bj][2], acc[bi][bj][3]};
+      float* p = F + row + static_cast<int64_t>(col) * m;
+      if (row + 3 < d.m) {
+        *reinterpret_cast<f4u*>(p) = o;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row + r < d.m) p[r] = o[r];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_q(
+    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles, float* __restrict__ arena,
+    const double* __restrict__ dpiv) {
+  gemm_tile_q(fd, tiles, arena, dpiv);
 }
 
 // The same tile under two register budgets: k_gemm asks for two waves per

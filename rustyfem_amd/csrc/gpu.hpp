@@ -21,7 +21,7 @@ struct Timings {
   double gemm_ms = 0, gemm_launches = 0, gemm_flops = 0;
   // GEMM autotuning at the first factorization: its time, the GEMM launches
   // of the plan and how many of them run the AGPR-accumulator variant
-  double tune_ms = 0, tuned_launches = 0, agpr_launches = 0, ilv_launches = 0;
+  double tune_ms = 0, tuned_launches = 0, agpr_launches = 0, ilv_launches = 0, q_launches = 0;
   // worst relative residual after 0, 1, 2, 3 refinement corrections (-1: not reached)
   double refine_history[4] = {-1, -1, -1, -1};
   // 1 when the last refinement took back a correction that raised the worst residual

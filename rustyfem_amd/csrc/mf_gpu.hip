@@ -20,6 +20,7 @@
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
+#include <type_traits>
 
 #include "gemm_kernel.hpp"
 #include "panel_kernels.hpp"
@@ -770,25 +771,63 @@ __global__ __launch_bounds__(256) void k_sfwd_chain(const FrontDesc* __restrict_
 }
 
 // t[r] -= L[r, J0:J1] y[J0:J1] for SRB rows; the four waves split the columns
-template <typename T, int R>
-__global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__ fd, const SbRowTile* __restrict__ tiles,
-                                                   const T* __restrict__ arena, const double* __restrict__ ysrc,
-                                                   double* __restrict__ t, int64_t tv, int nrhs) {
+// Waves per workgroup of the super-block solve kernels (k_sfwd_rows,
+// k_sfwd_inv, k_sbwd_comb, k_sbwd_inv): the columns of a tile are split among
+// the waves and every wave issues all loads of its share before its first
+// product, so that a launch waits for about one memory latency.  16 waves
+// with up to 4 right-hand sides per pass, 8 with 16 (LDS); levels whose
+// fronts have at most 128 pivots take 4 (many small tiles).
+template <int R>
+__host__ __device__ constexpr int sb_waves() {
+  return R > 4 ? 8 : 16;
+}
+// widest super-block a workgroup of NW waves serves (the 4-wave kernels are
+// launched only for levels whose fronts have at most 128 pivots)
+template <int NW>
+__host__ __device__ constexpr int sb_maxw() {
+  return NW == 4 ? 128 : SBW;
+}
+
+// sum of red[0..NW)[lane][j] in a fixed pairwise order
+template <int NW, int ROWS, int RP>
+__device__ __forceinline__ double wave_tree_sum(const double (&red)[NW][ROWS][RP], int lane, int j) {
+  double v[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) v[q] = red[q][lane][j];
+#pragma unroll
+  for (int h = NW / 2; h >= 1; h /= 2)
+#pragma unroll
+    for (int q = 0; q < h; ++q) v[q] = v[q] + v[q + h];
+  return v[0];
+}
+
+template <typename T, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sfwd_rows(const FrontDesc* __restrict__ fd,
+                                                                 const SbRowTile* __restrict__ tiles,
+                                                                 const T* __restrict__ arena,
+                                                                 const double* __restrict__ ysrc,
+                                                                 double* __restrict__ t, int64_t tv, int nrhs) {
+  constexpr int CPW = sb_maxw<NW>() / NW;
   const SbRowTile tl = tiles[blockIdx.x];
   const FrontDesc d = fd[tl.front];
   __shared__ double y[SBW][R];
-  __shared__ double red[4][SRB][R + 1];
+  __shared__ double red[NW][SRB][R + 1];
   const int W = tl.J1 - tl.J0, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int r = tl.r0 + lane;
   const bool valid = r < d.m;
-  const int cw = (W + 3) / 4, cb = w * cw, ce = min(W, cb + cw);
+  const int cw = (W + NW - 1) / NW, cb = w * cw, ce = min(W, cb + cw);
   double* tf = t + d.toff;
   const double* yf = ysrc + d.toff;
-  const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + r;
+  // the lane's row (clamped into the front) of the wave's columns
+  const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + min(r, d.m - 1);
   for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
+    // all of the wave's loads of L first (one memory latency), then y
+    T lv[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) lv[c] = F[static_cast<int64_t>(min(cb + c, W - 1)) * d.m];
     __syncthreads();
-    for (int idx = tid; idx < W * R; idx += 256) {
+    for (int idx = tid; idx < W * R; idx += 64 * NW) {
       const int i = idx % W, j = idx / W;
       y[i][j] = j < nr ? yf[(j0 + j) * tv + tl.J0 + i] : 0.0;
     }
@@ -796,20 +835,18 @@ __global__ __launch_bounds__(256) void k_sfwd_rows(const FrontDesc* __restrict__
     double acc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
-    if (valid) {
-#pragma unroll 16
-      for (int c = cb; c < ce; ++c) {
-        const double l = static_cast<double>(F[static_cast<int64_t>(c) * d.m]);
 #pragma unroll
-        for (int j = 0; j < R; ++j) acc[j] = fma(l, y[c][j], acc[j]);
+    for (int c = 0; c < CPW; ++c)
+      if (cb + c < ce) {
+        const double l = static_cast<double>(lv[c]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(l, y[cb + c][j], acc[j]);
       }
-    }
 #pragma unroll
     for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
     __syncthreads();
     if (tid < SRB && valid)
-      for (int j = 0; j < nr; ++j)
-        tf[(j0 + j) * tv + r] -= (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
+      for (int j = 0; j < nr; ++j) tf[(j0 + j) * tv + r] -= wave_tree_sum<NW>(red, lane, j);
   }
 }
 
@@ -1075,28 +1112,35 @@ __global__ __launch_bounds__(256) void k_sbinv(const FrontDesc* __restrict__ fd,
 
 // forward: y_J = X t_J for rows [64a, 64a + 64) of the super-block (a = blockIdx.y);
 // the result goes to ys (t_J is read by the other row blocks)
-template <int R>
-__global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
-                                                  const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
-                                                  const double* __restrict__ t, double* __restrict__ ys, int64_t tv,
-                                                  int nrhs) {
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sfwd_inv(const FrontDesc* __restrict__ fd,
+                                                                const SbTask* __restrict__ tasks,
+                                                                const double* __restrict__ xinv,
+                                                                const int64_t* __restrict__ sboff,
+                                                                const double* __restrict__ t, double* __restrict__ ys,
+                                                                int64_t tv, int nrhs) {
+  constexpr int CPW = sb_maxw<NW>() / NW;
   const SbTask tk = tasks[blockIdx.x];
   const int W = tk.J1 - tk.J0, a0 = blockIdx.y * NB;
   if (a0 >= W) return;
   const FrontDesc d = fd[tk.front];
   __shared__ double tJ[SBW][R];
-  __shared__ double red[4][NB][R + 1];
+  __shared__ double red[NW][NB][R + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = a0 + lane;
   const int jend = min(W, a0 + NB);          // X is lower triangular
-  const int cw = (jend + 3) / 4, cb = w * cw, ce = min(jend, cb + cw);
+  const int cw = (jend + NW - 1) / NW, cb = w * cw, ce = min(jend, cb + cw);
   const double* X = xinv + sboff[tk.sb];
   const double* tf = t + d.toff + tk.J0;
   double* yf = ys + d.toff + tk.J0;
+  const double* xr = X + min(i, W - 1);
   for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
+    double xv[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, jend - 1)) * W];
     __syncthreads();
-    for (int idx = tid; idx < jend * R; idx += 256) {
+    for (int idx = tid; idx < jend * R; idx += 64 * NW) {
       const int q = idx % jend, j = idx / jend;
       tJ[q][j] = j < nr ? tf[(j0 + j) * tv + q] : 0.0;
     }
@@ -1104,39 +1148,37 @@ __global__ __launch_bounds__(256) void k_sfwd_inv(const FrontDesc* __restrict__ 
     double acc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
-    if (i < W) {
-      const double* xr = X + i;
-#pragma unroll 8
-      for (int q = cb; q < ce; ++q) {
-        const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
-        for (int j = 0; j < R; ++j) acc[j] = fma(x, tJ[q][j], acc[j]);
+    for (int c = 0; c < CPW; ++c)
+      if (cb + c < ce) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(xv[c], tJ[cb + c][j], acc[j]);
       }
-    }
 #pragma unroll
     for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
     __syncthreads();
     if (tid < NB && i < W)
-      for (int j = 0; j < nr; ++j)
-        yf[(j0 + j) * tv + i] = (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
+      for (int j = 0; j < nr; ++j) yf[(j0 + j) * tv + i] = wave_tree_sum<NW>(red, lane, j);
   }
 }
 
 // backward: z_J = t_J - (partial sums in row-tile order) -> ys; 64 columns per
 // block (blockIdx.y), the four waves sum contiguous quarters of the row tiles
-template <int R>
-__global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
-                                                   const double* __restrict__ partial, const double* __restrict__ t,
-                                                   double* __restrict__ ys, int64_t tv, int nrhs) {
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sbwd_comb(const FrontDesc* __restrict__ fd,
+                                                                 const SbTask* __restrict__ tasks,
+                                                                 const double* __restrict__ partial,
+                                                                 const double* __restrict__ t,
+                                                                 double* __restrict__ ys, int64_t tv, int nrhs) {
   const SbTask tk = tasks[blockIdx.x];
   const int W = tk.J1 - tk.J0, c0 = blockIdx.y * NB;
   if (c0 >= W) return;
   const FrontDesc d = fd[tk.front];
-  __shared__ double red[4][NB][R + 1];
+  __shared__ double red[NW][NB][R + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = c0 + lane;
   const int cc = i / BC, cl = i % BC;
-  const int qw = (tk.nrc + 3) / 4, rb = w * qw, re = min(tk.nrc, rb + qw);
+  const int qw = (tk.nrc + NW - 1) / NW, rb = w * qw, re = min(tk.nrc, rb + qw);
   const double* tf = t + d.toff + tk.J0;
   double* zf = ys + d.toff + tk.J0;
   for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
@@ -1146,8 +1188,8 @@ __global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
     if (i < W) {
       const double* src = partial + ((static_cast<int64_t>(tk.p0) + static_cast<int64_t>(cc) * tk.nrc) * nrhs + j0) * BC + cl;
-      // the loads of eight row chunks are issued together
-#pragma unroll 8
+      // the loads of sixteen row chunks are issued together
+#pragma unroll 16
       for (int rc = rb; rc < re; ++rc)
 #pragma unroll
         for (int j = 0; j < R; ++j)
@@ -1158,36 +1200,41 @@ __global__ __launch_bounds__(256) void k_sbwd_comb(const FrontDesc* __restrict__
     for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
     __syncthreads();
     if (tid < NB && i < W)
-      for (int j = 0; j < nr; ++j)
-        zf[(j0 + j) * tv + i] =
-            tf[(j0 + j) * tv + i] - ((red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]));
+      for (int j = 0; j < nr; ++j) zf[(j0 + j) * tv + i] = tf[(j0 + j) * tv + i] - wave_tree_sum<NW>(red, lane, j);
   }
 }
 
 // backward: x_J = X^T z_J for rows [64a, 64a + 64) (a = blockIdx.y) from the
 // stored transpose: lane = output row, the four waves split the columns
-template <int R>
-__global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ fd, const SbTask* __restrict__ tasks,
-                                                  const double* __restrict__ xinvt, const int64_t* __restrict__ sboff,
-                                                  const double* __restrict__ ys, double* __restrict__ t, int64_t tv,
-                                                  int nrhs) {
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sbwd_inv(const FrontDesc* __restrict__ fd,
+                                                                const SbTask* __restrict__ tasks,
+                                                                const double* __restrict__ xinvt,
+                                                                const int64_t* __restrict__ sboff,
+                                                                const double* __restrict__ ys, double* __restrict__ t,
+                                                                int64_t tv, int nrhs) {
+  constexpr int CPW = sb_maxw<NW>() / NW;
   const SbTask tk = tasks[blockIdx.x];
   const int W = tk.J1 - tk.J0, a0 = blockIdx.y * NB;
   if (a0 >= W) return;
   const FrontDesc d = fd[tk.front];
   __shared__ double zs[SBW][R];
-  __shared__ double red[4][NB][R + 1];
+  __shared__ double red[NW][NB][R + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = a0 + lane;
   const int span = W - a0;                      // X^T is upper triangular: columns q >= a0
-  const int cw = (span + 3) / 4, cb = a0 + w * cw, ce = min(W, cb + cw);
+  const int cw = (span + NW - 1) / NW, cb = a0 + w * cw, ce = min(W, cb + cw);
   const double* Xt = xinvt + sboff[tk.sb];
   const double* zf = ys + d.toff + tk.J0;
   double* tf = t + d.toff + tk.J0;
+  const double* xr = Xt + min(i, W - 1);
   for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
+    double xv[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, W - 1)) * W];
     __syncthreads();
-    for (int idx = tid; idx < span * R; idx += 256) {
+    for (int idx = tid; idx < span * R; idx += 64 * NW) {
       const int q = a0 + idx % span, j = idx / span;
       zs[q][j] = j < nr ? zf[(j0 + j) * tv + q] : 0.0;
     }
@@ -1195,21 +1242,17 @@ __global__ __launch_bounds__(256) void k_sbwd_inv(const FrontDesc* __restrict__ 
     double acc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = 0.0;
-    if (i < W) {
-      const double* xr = Xt + i;
-#pragma unroll 8
-      for (int q = cb; q < ce; ++q) {
-        const double x = xr[static_cast<int64_t>(q) * W];
 #pragma unroll
-        for (int j = 0; j < R; ++j) acc[j] = fma(x, zs[q][j], acc[j]);
+    for (int c = 0; c < CPW; ++c)
+      if (cb + c < ce) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(xv[c], zs[cb + c][j], acc[j]);
       }
-    }
 #pragma unroll
     for (int j = 0; j < R; ++j) red[w][lane][j] = acc[j];
     __syncthreads();
     if (tid < NB && i < W)
-      for (int j = 0; j < nr; ++j)
-        tf[(j0 + j) * tv + i] = (red[0][lane][j] + red[1][lane][j]) + (red[2][lane][j] + red[3][lane][j]);
+      for (int j = 0; j < nr; ++j) tf[(j0 + j) * tv + i] = wave_tree_sum<NW>(red, lane, j);
   }
 }
 
@@ -1458,13 +1501,15 @@ class Impl final : public GpuFactor {
       // RUSTYFEM_TUNE_MARGIN: a variant must be this fraction faster to be picked
       const char* tm = std::getenv("RUSTYFEM_TUNE_MARGIN");
       if (tm && *tm) tune_margin_ = 1.0f - static_cast<float>(std::atof(tm));
-      // RUSTYFEM_AUTOTUNE: unset or 1 = time the four GEMM kernels per
-      // launch, 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere,
-      // "ilv" = k_gemm_ilv everywhere, "k64" = k_gemm_k64 everywhere
+      // RUSTYFEM_AUTOTUNE: unset or 1 = time every GEMM kernel per launch,
+      // 0 = k_gemm everywhere, "agpr" = k_gemm_agpr everywhere, "ilv" =
+      // k_gemm_ilv everywhere, "k64" = k_gemm_k64 everywhere, "q" = k_gemm_q
+      // everywhere (FP32; FP64 factors take k_gemm)
       const char* a = std::getenv("RUSTYFEM_AUTOTUNE");
       const int8_t force = a && std::strcmp(a, "agpr") == 0  ? 1
                            : a && std::strcmp(a, "ilv") == 0 ? 2
                            : a && std::strcmp(a, "k64") == 0 ? 3
+                           : a && std::strcmp(a, "q") == 0   ? (kGemmVariants > 4 ? 4 : 0)
                                                              : 0;
       autotune_ = !(a && *a == '0') && force == 0;
       gemm_pick_.assign(P_.gemm.size(), force);
@@ -1487,7 +1532,9 @@ class Impl final : public GpuFactor {
           slot_maxu_[sl][L] = std::max<int64_t>(slot_maxu_[sl][L], P_.fd[P_.child_slot_fronts[sl][q]].u);
       }
     }
-    d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems), bytes_);
+    // 64 entries of padding: k_gemm_q's 16-byte loads of a quad of rows that
+    // starts at a front's last row read up to three entries past its last column
+    d_arena_ = dalloc<T>(static_cast<size_t>(P_.arena_elems) + 64, bytes_);
     d_dpiv_ = dalloc<double>(n_, bytes_);
     d_adiag_ = dalloc<double>(n_, bytes_);
     d_lval_ = dalloc<double>(nlower_, bytes_);
@@ -1853,8 +1900,16 @@ class Impl final : public GpuFactor {
     }
   }
 
-  // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv, 3 = k_gemm_k64
+  // variant 0 = k_gemm, 1 = k_gemm_agpr, 2 = k_gemm_ilv, 3 = k_gemm_k64,
+  // 4 = k_gemm_q (FP32 only)
+  static constexpr int kGemmVariants = std::is_same<T, float>::value ? 5 : 4;
   void launch_gemm(int variant, const Range& r, hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (variant == 4) {
+        launch_on(st, k_gemm_q, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+        return;
+      }
+    }
     if (variant == 1)
       launch_on(st, k_gemm_agpr<T>, r.count, 256, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
     else if (variant == 2)
@@ -1879,7 +1934,7 @@ class Impl final : public GpuFactor {
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
         if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0) ph.push_back(&p);
-    constexpr int NV = 4;
+    constexpr int NV = kGemmVariants;
     std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(e0_, s_));
@@ -1895,7 +1950,7 @@ class Impl final : public GpuFactor {
     }
     HIPCHK(hipEventRecord(e1_, s_));
     HIPCHK(hipStreamSynchronize(s_));
-    int64_t count[NV] = {0, 0, 0, 0};
+    int64_t count[NV] = {};
     for (size_t i = 0; i < ph.size(); ++i) {
       float best[NV];
       for (int v = 0; v < NV; ++v) {
@@ -1919,6 +1974,7 @@ class Impl final : public GpuFactor {
     tm_.tuned_launches = static_cast<double>(ph.size());
     tm_.agpr_launches = static_cast<double>(count[1]);
     tm_.ilv_launches = static_cast<double>(count[2] + count[3]);
+    tm_.q_launches = NV > 4 ? static_cast<double>(count[NV - 1]) : 0.0;
     tuned_ = true;
   }
 
@@ -2141,10 +2197,47 @@ class Impl final : public GpuFactor {
     }
   }
 
+  // the super-block steps of one level, NW waves per workgroup
+  template <int R, int NW>
+  void sb_fwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
+    const int nbk = SBW / NB;
+    for (const SbFwdStep& st : lp.sfwd) {
+      if (st.chain.count > 0) {
+        hipLaunchKernelGGL((k_sfwd_inv<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
+                           s_, d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, w_.t, w_.ys, tv, nrhs);
+        HIPCHK(hipGetLastError());
+      }
+      if (st.rows.count > 0) {
+        hipLaunchKernelGGL((k_sfwd_rows<T, R, NW>), dim3(grid(st.rows.count), nch), dim3(64 * NW), 0, s_, d_fd_,
+                           d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+
+  template <int R, int NW>
+  void sb_bwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
+    const int nbk = SBW / NB;
+    for (const SbBwdStep& st : lp.sbwd) {
+      if (st.part.count > 0) {
+        hipLaunchKernelGGL((k_sbwd_part<T, R>), dim3(grid(st.part.count), nch), dim3(256), 0, s_, d_fd_,
+                           d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
+        HIPCHK(hipGetLastError());
+      }
+      if (st.chain.count > 0) {
+        hipLaunchKernelGGL((k_sbwd_comb<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
+                           s_, d_fd_, d_sbt_ + st.chain.begin, w_.part, w_.t, w_.ys, tv, nrhs);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((k_sbwd_inv<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
+                           s_, d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, w_.ys, w_.t, tv, nrhs);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+
   template <int R>
   void solve_levels_inv(double* dx, int nrhs) {
     const int64_t tv = P_.tvec_elems;
-    const int nbk = SBW / NB;
     const unsigned nch = static_cast<unsigned>((nrhs + R - 1) / R);  // right-hand-side chunks, one per workgroup
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
@@ -2155,41 +2248,20 @@ class Impl final : public GpuFactor {
         launch_vec(k_fwd_child, cr.count, nrhs, slot_maxu_[sl][L], d_fd_, d_cs_[sl] + cr.begin, d_cmap_, w_.t, tv,
                    nrhs);
       }
-      for (const SbFwdStep& st : lp.sfwd) {
-        if (st.chain.count > 0) {
-          hipLaunchKernelGGL(k_sfwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, w_.t, w_.ys, tv, nrhs);
-          HIPCHK(hipGetLastError());
-        }
-        if (st.rows.count > 0) {
-          auto* kr = &k_sfwd_rows<T, R>;
-          hipLaunchKernelGGL(kr, dim3(grid(st.rows.count), nch), dim3(256), 0, s_, d_fd_, d_sbrow_ + st.rows.begin,
-                             d_arena_, w_.ys, w_.t, tv, nrhs);
-          HIPCHK(hipGetLastError());
-        }
-      }
+      if (lvl_maxk_[L] <= 128)
+        sb_fwd_steps<R, 4>(lp, nrhs, tv, nch);
+      else
+        sb_fwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch);
       launch_vec(k_fwd_finish_ys, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.ys, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
       launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
-      for (const SbBwdStep& st : lp.sbwd) {
-        if (st.part.count > 0) {
-          auto* kp = &k_sbwd_part<T, R>;
-          hipLaunchKernelGGL(kp, dim3(grid(st.part.count), nch), dim3(256), 0, s_, d_fd_, d_sbpart_ + st.part.begin,
-                             d_arena_, w_.t, tv, nrhs, w_.part);
-          HIPCHK(hipGetLastError());
-        }
-        if (st.chain.count > 0) {
-          hipLaunchKernelGGL(k_sbwd_comb<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, w_.part, w_.t, w_.ys, tv, nrhs);
-          HIPCHK(hipGetLastError());
-          hipLaunchKernelGGL(k_sbwd_inv<R>, dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(256), 0, s_,
-                             d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, w_.ys, w_.t, tv, nrhs);
-          HIPCHK(hipGetLastError());
-        }
-      }
+      if (lvl_maxk_[L] <= 128)
+        sb_bwd_steps<R, 4>(lp, nrhs, tv, nch);
+      else
+        sb_bwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch);
       launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, dx, n_, tv, nrhs);
     }
   }
@@ -2213,7 +2285,8 @@ class Impl final : public GpuFactor {
       for (const SbFwdStep& st : lp.sfwd) {
         launch(k_sfwd_chain<T, R>, st.chain.count, 256, d_fd_, d_sbt_ + st.chain.begin, d_arena_, d_vbuf_, d_btask_,
                w_.t, tv, nrhs);
-        launch(k_sfwd_rows<T, R>, st.rows.count, 256, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.t, w_.t, tv, nrhs);
+        launch(k_sfwd_rows<T, R, sb_waves<R>()>, st.rows.count, 64 * sb_waves<R>(), d_fd_, d_sbrow_ + st.rows.begin,
+               d_arena_, w_.t, w_.t, tv, nrhs);
       }
       launch_vec(k_fwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, d_dpiv_, dx, n_, tv, nrhs);
     }

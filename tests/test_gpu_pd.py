@@ -191,9 +191,10 @@ def test_inverse_and_chain_superblock_solves_agree(rf, monkeypatch):
 
 @pytest.mark.parametrize("precision", [None, "mixed"])
 def test_gemm_register_layouts_agree(rf, monkeypatch, precision):
-    """The four Schur GEMM kernels (k_gemm, k_gemm_agpr, k_gemm_ilv, k_gemm_k64)
-    run the same arithmetic: forcing any one gives identical D and solutions,
-    and the autotuner times every GEMM launch of the plan."""
+    """The Schur GEMM kernels (k_gemm, k_gemm_agpr, k_gemm_ilv, k_gemm_k64 and,
+    in FP32, k_gemm_q) run the same arithmetic: forcing any one gives
+    identical D and solutions, and the autotuner times every GEMM launch of
+    the plan."""
     p = meshes.hex_block(12, nrhs=2, seed=5)
     monkeypatch.setenv("RUSTYFEM_AUTOTUNE", "0")
     h_v, _, d_v = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
@@ -203,16 +204,21 @@ def test_gemm_register_layouts_agree(rf, monkeypatch, precision):
     h_i, _, d_i = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
     monkeypatch.setenv("RUSTYFEM_AUTOTUNE", "k64")
     h_k, _, d_k = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+    monkeypatch.setenv("RUSTYFEM_AUTOTUNE", "q")
+    h_q, _, d_q = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
     monkeypatch.delenv("RUSTYFEM_AUTOTUNE")
     h_t, _, d_t = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
     assert np.array_equal(d_a, d_v) and np.array_equal(d_i, d_v) and np.array_equal(d_k, d_v)
+    assert np.array_equal(d_q, d_v)
     assert np.array_equal(d_t, d_v)
     x = h_v.solve(p.rhs)
-    for h in (h_a, h_i, h_k, h_t):
+    for h in (h_a, h_i, h_k, h_q, h_t):
         assert np.array_equal(h.solve(p.rhs), x)
     st = h_t.stats()
     assert st["tuned_launches"] > 0
-    assert 0 <= st["agpr_launches"] + st["ilv_launches"] <= st["tuned_launches"]
+    assert 0 <= st["agpr_launches"] + st["ilv_launches"] + st["q_launches"] <= st["tuned_launches"]
+    if precision is None:
+        assert st["q_launches"] == 0
     assert h_v.stats()["tuned_launches"] == 0 and h_i.stats()["tuned_launches"] == 0
 
 
