@@ -524,6 +524,58 @@ def run_batch(args):
     print(json.dumps(batch_result(args)), flush=True)
 
 
+def indef_result(args):
+    """The reference's indefinite benchmark (examples/solver_bench.rs:87-157):
+    the 7-point Laplacian on a g^3 grid shifted by sigma midway between two
+    eigenvalues near the 10 % quantile, factored by the indefinite LDL^T
+    (1x1/2x2 pivots, delayed pivots, Bunch-Kaufman at the roots).  One step =
+    the numeric factorization from the host values (the pattern's analysis is
+    done once, as the reference times indef::factor after sparse::analyse).
+    Checks: the inertia against the analytic count (51,205 at 80^3,
+    docs/REFERENCE.md:419-422) and the solve residual."""
+    import numpy as np
+    from rustyfem_amd import ffi, meshes
+    ffi.lib()
+    g = args.indef_size
+    t0 = time.perf_counter()
+    sigma, below = meshes.laplacian_sigma(g)
+    p = meshes.laplacian3d(g, sigma)
+    log(f"shifted Laplacian {g}^3: n={p.n} sigma={sigma:.6f} analytic negatives {below}, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    h, err, ine = ffi.Handle.factor_indef(p)
+    if h is None:
+        raise RuntimeError(f"indefinite factorization failed: err={err} {ffi.last_error()}")
+    first = time.perf_counter() - t0
+    times = []
+    for i in range(max(1, min(args.steps, 3))):
+        t0 = time.perf_counter()
+        if h.refactor(p.data) != -1:
+            raise RuntimeError("indefinite refactorization failed")
+        times.append(time.perf_counter() - t0)
+        log(f"indef {g}^3: factorization {times[-1]:.2f}s")
+    ine2 = h.inertia()
+    x = h.solve(p.rhs)
+    rel = float(np.linalg.norm(p.rhs[:, 0] - p.to_scipy() @ x[:, 0]) / np.linalg.norm(p.rhs[:, 0]))
+    st = h.factor_stats()
+    t = float(np.median(times))
+    out = {"value": t, "unit": "s", "ms_per_step": t * 1e3, "dtype": "fp64",
+           "config": {"workload": f"shifted 3D Laplacian {g}^3 (n={p.n}), sigma={sigma:.6f} at the 10 % eigenvalue "
+                                  f"quantile; indefinite LDL^T with delayed pivots, ND leaf 16",
+                      "n": p.n, "sigma": sigma},
+           "inertia": [int(v) for v in ine2], "analytic_negatives": below,
+           "inertia_matches_analytic": bool(int(ine2[0]) == below and int(ine2[1]) == 0),
+           "first_factor_with_analysis_s": first, "true_rel_residual": rel,
+           "delayed_pivots": int(st[3]), "two_by_two_pivots": int(st[4]), "factor_entries": int(st[5]),
+           "reference_published_s": 44.0 if g == 80 else None,
+           "reference_published": "docs/REFERENCE.md:419-422 (CPU, 512,000 unknowns, 51,205 negative pivots)"}
+    return out
+
+
+def run_indef(args):
+    print(json.dumps(indef_result(args)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,7 +590,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="direct workload only: skip the batch and learned-smoother sub-benchmarks")
-    ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned", "ic0"], default="direct",
+    ap.add_argument("--indef-size", type=int, default=80, help="grid edge of the shifted Laplacian (indef workload)")
+    ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned", "ic0", "indef"], default="direct",
                     help="direct: the multifrontal block (default); amg: the heterogeneous block with AMG-Krylov; "
                          "batch: 512 plates factored and solved together; learned: the C3D4 block with the "
                          "learned-smoother FCG")
@@ -563,6 +616,9 @@ def main():
         return run_ic0(args)
     if args.workload == "learned":
         run_learned(args)
+        return
+    if args.workload == "indef":
+        run_indef(args)
         return
     os.environ.setdefault("RUSTYFEM_KERNEL_TIMING", "1")
     import numpy as np

@@ -107,6 +107,24 @@ int32_t rustyfem_solve_device(const Handle *h, size_t nrhs, double *d_b);
  * would allocate): the call returns 1 instead.  0 on success. */
 int32_t rustyfem_solve_device_noalloc(const Handle *h, size_t nrhs, double *d_b);
 
+/* Shift-invert block subspace iteration with Rayleigh-Ritz (additive; the
+ * reference's src/solver/eigen.rs:229-375 `subspace`, a consumer of the
+ * factorization).  h holds the positive definite factorization of
+ * Op = K - sigma M; B (= M) is symmetric n x n, given in CSC with both
+ * triangles in the caller's numbering.  Finds the nev values theta of largest
+ * magnitude of Op x = (1/theta) B x, i.e. the eigenvalues
+ * lambda = sigma + 1/theta of K x = lambda M x closest to sigma, with
+ * nvec iteration vectors (0 = min(2 nev, nev + 8)), at most max_iter
+ * iterations (0 = 2000), until every requested Ritz vector has
+ * ||Op^-1 B x - theta x||_Op <= tol |theta| (tol 0 = 1e-10).  theta_out
+ * (nev entries, largest |theta| first); x_out (n x nev, column-major, may be
+ * NULL): vectors with x^T Op x = 1; iterations_out may be NULL.  Returns the
+ * number of pairs found (fewer than nev when B has lower rank), -1 on an
+ * error (rustyfem_last_error), including no convergence. */
+int64_t rustyfem_eigen(const Handle *h, const int64_t *b_indptr, const int64_t *b_indices, const double *b_data,
+                       size_t nev, size_t nvec, size_t max_iter, double tol, double *theta_out, double *x_out,
+                       int64_t *iterations_out);
+
 /* Statistics of the last factorization / solve (additive):
  * out[0] factor ms, [1] solve ms, [2] refined-solve ms, [3] refinement steps,
  * [4] relative residual after refinement (-1 if not refined), [5] flops,

@@ -96,6 +96,9 @@ def lib():
             "ora_amg_free": (None, [vp]),
             "ora_rsort_net9": (None, [i64p, vp]),
             "ora_rsort_net13": (None, [i64p, vp]),
+            "ora_eigen": (C.c_int64, [vp, sz, i64p, i64p, f64p, sz, sz, sz, C.c_double, f64p, vp,
+                                      C.POINTER(C.c_int64)]),
+            "ora_jacobi_eigen": (None, [sz, f64p, f64p, f64p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -390,6 +393,21 @@ class Handle:
         assert rc == 0
         return flat.reshape(x.shape, order="F")
 
+    def eigen(self, b, nev, nvec=0, max_iter=2000, tol=1e-10):
+        """Shift-invert subspace iteration (src/solver/eigen.rs:258-387) on this
+        PD handle (Op) with B a scipy sparse symmetric matrix: (theta, x, iterations)."""
+        import scipy.sparse as sp
+        bc = sp.csc_matrix(b)
+        bc.sort_indices()
+        th = np.zeros(max(int(nev), 1))
+        x = np.zeros(self.n * max(int(nev), 1))
+        its = C.c_int64(0)
+        k = lib().ora_eigen(self._h, self.n, _i64(bc.indptr), _i64(bc.indices), _f64(bc.data), int(nev), int(nvec),
+                            int(max_iter), float(tol), th, x.ctypes.data, C.byref(its))
+        if k < 0:
+            raise RuntimeError(f"ora_eigen failed ({k})")
+        return th[:k], x[:self.n * k].reshape((self.n, k), order="F"), int(its.value)
+
     def inertia(self):
         o = np.zeros(3, dtype=np.int64)
         rc = lib().ora_ffi_inertia(self._h, o)
@@ -422,6 +440,16 @@ class Handle:
 
     def sym(self, nn):
         return Symbolic._borrow(lib().ora_ffi_sym(self._h), nn)
+
+
+def jacobi_eigen(a):
+    """The oracle's cyclic Jacobi (src/solver/eigen.rs:159-214): (d, v)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    d = np.zeros(max(n, 1))
+    v = np.zeros(max(n * n, 1))
+    lib().ora_jacobi_eigen(n, a.reshape(-1), d, v)
+    return d[:n], v[:n * n].reshape(n, n)
 
 
 def solve_lower(n, colptr, rowidx, val, b, kind="amg", coords=None, ndof=None, tol=1e-10, maxit=0):

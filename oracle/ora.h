@@ -115,6 +115,16 @@ const ora_sym *ora_ffi_sym(const ora_handle *h);
 size_t ora_ffi_leaf(void);
 void ora_set_ffi_leaf(size_t leaf);
 
+/* ---- eigenvalues (src/solver/eigen.rs) ---- */
+/* shift-invert block subspace iteration on a PD handle holding Op; B in CSC
+ * with both triangles; returns the pairs found, -1 (no convergence) or -2 */
+int64_t ora_eigen(const ora_handle *h, size_t n, const int64_t *bp, const int64_t *bi, const double *bv, size_t nev,
+                  size_t vectors, size_t max_iter, double tol, double *theta_out, double *x_out,
+                  int64_t *iterations);
+/* cyclic Jacobi of a dense symmetric n x n matrix (row-major); v[i n + j] =
+ * component i of eigenvector j */
+void ora_jacobi_eigen(size_t n, const double *a, double *d, double *v);
+
 /* ---- iterative path (src/solver/iterative*.rs) ---- */
 /* kind: 0 AMG, 1 IC(0), 2 Jacobi.  Returns iterations; rel (relative recursive residual), converged. */
 int64_t ora_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,

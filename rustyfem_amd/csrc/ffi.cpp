@@ -18,6 +18,7 @@
 
 #include "../../include/rustyfem.h"
 #include "amg.hpp"
+#include "eigen.hpp"
 #include "gpu.hpp"
 #include "indef.hpp"
 #include "mf.hpp"
@@ -649,6 +650,42 @@ int32_t rustyfem_solve_device_noalloc(const Handle* h, size_t nrhs, double* d_b)
     std::lock_guard<std::mutex> lock(h->solve_mu);
     const bool met = h->gpu->solve_noalloc(d_b, static_cast<int>(nrhs), h->opt.refine_tol, h->opt.max_refine);
     return met ? 0 : 1;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+int64_t rustyfem_eigen(const Handle* h, const int64_t* b_indptr, const int64_t* b_indices, const double* b_data,
+                       size_t nev, size_t nvec, size_t max_iter, double tol, double* theta_out, double* x_out,
+                       int64_t* iterations_out) {
+  try {
+    if (h->mode != Mode::Pd) throw std::runtime_error("the eigenvalue solver needs a positive definite handle");
+    const int64_t n = h->n;
+    for (int64_t c = 0; c < n; ++c)
+      if (b_indptr[c + 1] < b_indptr[c]) throw std::runtime_error("B: column pointers decrease");
+    for (int64_t p = 0; p < b_indptr[n]; ++p)
+      if (b_indices[p] < 0 || b_indices[p] >= n) throw std::runtime_error("B: row index out of range");
+    rf::EigenOptions o;
+    o.nev = static_cast<int>(nev);
+    o.vectors = static_cast<int>(nvec);
+    if (max_iter > 0) o.max_iter = static_cast<int>(max_iter);
+    if (tol > 0.0) o.tol = tol;
+    std::lock_guard<std::mutex> lock(h->solve_mu);
+    // Op^-1 on device blocks in the caller's numbering (the handle's solve,
+    // with its refinement when the handle is mixed precision)
+    const rf::BlockSolve solve = [h](double* d_y, int q) {
+      h->gpu->solve(d_y, q, true, h->opt.refine_tol, h->opt.max_refine);
+    };
+    // a symmetric B's CSC arrays are its CSR arrays
+    const rf::EigenResult r = rf::subspace_gpu(n, b_indptr, b_indices, b_data, solve, o);
+    const size_t k = r.theta.size();
+    std::copy(r.theta.begin(), r.theta.end(), theta_out);
+    if (x_out) std::copy(r.vectors.begin(), r.vectors.end(), x_out);
+    if (iterations_out) *iterations_out = r.iterations;
+    return static_cast<int64_t>(k);
   } catch (const std::exception& ex) {
     set_error(ex.what());
   } catch (...) {

@@ -26,7 +26,7 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch",
            "rustyfem_pcg_set_learned_smoother", "rustyfem_mpnn_nparams", "rustyfem_mpnn_create", "rustyfem_mpnn_loss",
            "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition", "rustyfem_chebyshev",
-           "rustyfem_pcg_setup_ex", "rustyfem_pcg_level", "rustyfem_amg_aggregate"]
+           "rustyfem_pcg_setup_ex", "rustyfem_pcg_level", "rustyfem_amg_aggregate", "rustyfem_eigen"]
 
 _lib = None
 
@@ -71,6 +71,8 @@ def lib():
         L.rustyfem_solve.argtypes = [vp, sz, vp]
         L.rustyfem_solve_device.restype = C.c_int32
         L.rustyfem_solve_device.argtypes = [vp, sz, vp]
+        L.rustyfem_eigen.restype = C.c_int64
+        L.rustyfem_eigen.argtypes = [vp, vp, vp, vp, sz, sz, sz, C.c_double, vp, vp, i64p]
         L.rustyfem_solve_device_noalloc.restype = C.c_int32
         L.rustyfem_solve_device_noalloc.argtypes = [vp, sz, vp]
         L.rustyfem_info.restype = None
@@ -268,6 +270,25 @@ class Handle:
         if rc < 0:
             raise RuntimeError("rustyfem_solve_device_noalloc failed: " + last_error())
         return rc == 0
+
+    def eigen(self, b, nev, nvec=0, max_iter=0, tol=0.0, vectors=True):
+        """rustyfem_eigen with B a scipy sparse symmetric matrix (caller numbering):
+        (theta, x or None, iterations)."""
+        import scipy.sparse as sp
+        bc = sp.csc_matrix(b)
+        bc.sort_indices()
+        ip = np.ascontiguousarray(bc.indptr, dtype=np.int64)
+        ix = np.ascontiguousarray(bc.indices, dtype=np.int64)
+        dd = np.ascontiguousarray(bc.data, dtype=np.float64)
+        th = np.zeros(max(int(nev), 1))
+        x = np.zeros((self.n, max(int(nev), 1)), order="F") if vectors else None
+        its = C.c_int64(0)
+        k = lib().rustyfem_eigen(self._h, ip.ctypes.data, ix.ctypes.data, dd.ctypes.data, int(nev), int(nvec),
+                                 int(max_iter), float(tol), th.ctypes.data, x.ctypes.data if vectors else None,
+                                 C.byref(its))
+        if k < 0:
+            raise RuntimeError("rustyfem_eigen failed: " + last_error())
+        return th[:k], (x[:, :k] if vectors else None), int(its.value)
 
     def refactor(self, data=None):
         err = C.c_int64(0)

@@ -639,3 +639,47 @@ def node_graph(p: Problem):
     np.add.at(adj_ptr, aa + 1, 1)
     adj_ptr = np.cumsum(adj_ptr)
     return nodes, p.coords[nodes].copy(), ndof, adj_ptr, bb.astype(np.int64)
+
+
+# ---------------------------------------------------------------------------
+# shifted 3D Laplacian (examples/solver_bench.rs:87-157): the reference's
+# indefinite benchmark problem
+
+
+def laplacian_sigma(g: int):
+    """sigma midway between two distinct eigenvalues of the g^3 7-point
+    Laplacian near its 10 % quantile, and the number of eigenvalues below it
+    (examples/solver_bench.rs:122-141: eigenvalues 6 - sum_a 2 cos(k_a pi/(g+1)))."""
+    c = 2.0 * np.cos(np.arange(1, g + 1) * math.pi / (g + 1))
+    lam = np.sort((((6.0 - c[:, None, None]) - c[None, :, None]) - c[None, None, :]).ravel())
+    q = g ** 3 // 10
+    while lam[q + 1] - lam[q] < 1e-9:
+        q += 1
+    sigma = 0.5 * (lam[q] + lam[q + 1])
+    return float(sigma), int(np.count_nonzero(lam < sigma))
+
+
+def laplacian3d(g: int, sigma: float = 0.0, nrhs: int = 1, seed: int = 5) -> Problem:
+    """The 7-point Laplacian on a g^3 grid (unit spacing, node i + g j + g^2 k at
+    (i, j, k), one DOF per node) shifted by sigma: diagonal 6 - sigma, -1 to
+    each grid neighbour (examples/solver_bench.rs:87-157); both triangles in
+    CSC, sorted rows.  Right-hand sides: seeded N(0, 1)."""
+    nn = g ** 3
+    k, j, i = np.meshgrid(np.arange(g), np.arange(g), np.arange(g), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    v = i + g * (j + g * k)
+    rows, cols, vals = [v], [v], [np.full(nn, 6.0 - sigma)]
+    for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+        ii, jj, kk = i + di, j + dj, k + dk
+        ok = (ii >= 0) & (ii < g) & (jj >= 0) & (jj < g) & (kk >= 0) & (kk < g)
+        rows.append(ii[ok] + g * (jj[ok] + g * kk[ok]))
+        cols.append(v[ok])
+        vals.append(np.full(int(ok.sum()), -1.0))
+    import scipy.sparse as sp
+    a = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nn, nn))
+    a.sort_indices()
+    coords = np.stack([i, j, k], axis=1).astype(np.float64)
+    rhs = np.asfortranarray(np.random.default_rng(seed).standard_normal((nn, nrhs)))
+    return Problem(name=f"laplace3d_{g}", n=nn, nn=nn, loc=np.arange(nn, dtype=np.int64), coords=coords,
+                   indptr=a.indptr.astype(np.int64), indices=a.indices.astype(np.int64), data=a.data.copy(), rhs=rhs,
+                   ndof=1, meta={"sigma": sigma})

@@ -114,6 +114,27 @@ def test_laplacian_inertia(rf, ora, g):
     assert np.max(np.abs(x - xo)) / np.max(np.abs(xo)) < 1e-8
 
 
+def test_laplacian_40_pivot_sequence_and_inertia(rf, ora):
+    """The reference's indefinite benchmark family at 40^3 (64,000 equations,
+    examples/solver_bench.rs:87-157): the pivot sequence equals the oracle's
+    bit for bit and the inertia is the analytic count of eigenvalues below
+    sigma (6,406)."""
+    import time
+    sigma, below = meshes.laplacian_sigma(40)
+    p = meshes.laplacian3d(40, sigma)
+    print(f"case laplacian g=40 n={p.n} sigma={sigma:.6f} analytic negatives {below}", flush=True)
+    t0 = time.perf_counter()
+    h, err, ine = rf.Handle.factor_indef(p)
+    print(f"GPU analysis + factorization {time.perf_counter() - t0:.2f}s", flush=True)
+    assert h is not None, rf.last_error()
+    assert list(ine) == [below, 0, p.n - below] and below == 6406
+    ho, _, ineo = ora.Handle.factor_indef(p)
+    assert list(ineo) == list(ine)
+    assert_same_factor(h, ho)
+    x = h.solve(p.rhs)
+    assert relres(p, x, p.rhs)[0] < 1e-9
+
+
 @pytest.mark.parametrize("g,dscale", [(5, 1e-3), (6, 1e-2), (8, 1e-3)])
 def test_delayed_and_2x2_pivots(rf, ora, g, dscale):
     """Bipartite grid with a tiny diagonal: 2x2 pivots and delayed pivots."""
