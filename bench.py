@@ -18,9 +18,10 @@ FP64 solution on that sample.
 The same line carries the other configurations as sub-objects, each with its
 own roofline, CPU baseline and error against the FP64 oracle: ``c2_16rhs``
 (the same block with 16 right-hand sides, both arms), ``c3_batch`` (512
-plates factored and solved together, 8 load cases each) and ``c5_learned``
-(the C3D4 block with the learned-smoother FCG against plain AMG-PCG).
-``--no-extras`` leaves the last two out.
+plates factored and solved together, 8 load cases each), ``c5_learned``
+(the C3D4 block with the learned-smoother FCG against plain AMG-PCG) and
+``f1_assembly`` (the block's stiffness assembled on the GPU, bit for bit
+against the oracle).  ``--no-extras`` leaves the last three out.
 """
 from __future__ import annotations
 
@@ -576,6 +577,108 @@ def run_indef(args):
     print(json.dumps(indef_result(args)), flush=True)
 
 
+def assembly_result(args):
+    """GPU element assembly of the benchmark block (C3D8 B-bar, size^3
+    elements, x = 0 clamped, E = 210e9, nu = 0.3 per element): one step =
+    the element-matrix kernel + the gather of every stored entry into device
+    memory, materials given on the host.  The pattern and contribution lists
+    are built once (setup_s).  Checked bit for bit against the oracle's
+    sequential restatement of the reference's element loop and scatter, which
+    is also the CPU baseline (one core; the reference's own scatter is
+    serial, src/fem/assembly.rs:378-398).  Also timed: the refactorization
+    fed straight from the assembly (rustyfem_assembly_refactor) against the
+    host round trip (values to the host, rustyfem_refactor)."""
+    import numpy as np
+    import torch
+    import oracle
+    from rustyfem_amd import ffi, meshes
+    ffi.lib()
+    n = args.size
+    kk, jj, ii = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    coords = np.stack([ii.ravel() / n, jj.ravel() / n, kk.ravel() / n], axis=1)
+    free = (ii > 0).ravel()
+    eq = meshes.equation_numbers(free, 3)
+    ndof = 3 * int(free.sum())
+    conn = meshes.grid_elements((n, n, n))
+    ne = conn.shape[0]
+    young, nu = np.full(ne, 210e9), np.full(ne, 0.3)
+    t0 = time.perf_counter()
+    a = ffi.Assembly("C3D8", conn, coords, eq, ndof)
+    setup = time.perf_counter() - t0
+    out_t = torch.empty(a.nnz, dtype=torch.float64, device="cuda")
+    steps = max(args.steps, 3)
+    for _ in range(max(args.warmup, 1)):
+        a.values(young, nu, out=out_t)
+    torch.cuda.synchronize()
+    ke_ms, ga_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a.values(young, nu, out=out_t)
+        st = a.stats()
+        ke_ms.append(st["element_ms"])
+        ga_ms.append(st["gather_ms"])
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / steps
+    gpu = out_t.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = oracle.assemble("C3D8", conn, coords, eq, ndof, young, nu, a.indptr, a.indices)
+    cpu_s = time.perf_counter() - t0
+    mismatches = int(np.count_nonzero(gpu != ref))
+    # algorithmic work: the B^T (D B) products of the 8 points, 2 flops per
+    # term over the structurally nonzero rows (3 normal rows of 24 columns,
+    # 3 shear rows of 16), and the gather's bytes (lists, element entries,
+    # output)
+    flops_elem = 2 * 8 * (3 * 24 * 24 + 3 * 16 * 16)
+    ncon = st["contributions"]
+    gbytes = 4 * (a.nnz + 1) + 4 * ncon + 8 * ncon + 8 * a.nnz
+    kms, gms = float(np.median(ke_ms)), float(np.median(ga_ms))
+    el_tflops = flops_elem * ne / (kms * 1e-3) / 1e12
+    ga_gbs = gbytes / (gms * 1e-3) / 1e9
+    res = {"value": step_s, "unit": "s", "ms_per_step": step_s * 1e3, "dtype": "fp64",
+           "config": {"workload": f"C3D8 B-bar stiffness assembly of the {n}^3 block (n={ndof} DOF, {ne} elements, "
+                                  f"nnz={a.nnz} both triangles); step = element kernel + gather into HBM",
+                      "elements": ne, "n": ndof, "nnz": a.nnz, "contributions": ncon},
+           "setup_s": setup, "element_kernel_ms": kms, "gather_kernel_ms": gms,
+           "roofline": {"bound": "valu", "achieved": el_tflops, "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": el_tflops / 78.6, "traffic": None,
+                        "kernel": f"k_elem<C3D8> (FP64 vector ALU, no FMA contraction): {flops_elem} flops per "
+                                  f"element (the B^T D B terms)"},
+           "gather_roofline": {"bound": "hbm", "achieved": ga_gbs, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                               "frac": ga_gbs / PEAK["hbm_gbs"], "traffic": None,
+                               "kernel": f"k_gather_sum: {gbytes} bytes per launch (offsets, contribution "
+                                         f"indices, element entries, output)"},
+           "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": 1, "kind": "port",
+                            "sample": f"the whole workload: the C oracle's sequential element loop and scatter "
+                                      f"({ne} elements)"},
+           "mismatches_vs_oracle": mismatches}
+    # assembly straight into the factorization against the host round trip
+    prob = meshes.Problem("asm", ndof, coords.shape[0], np.repeat(np.nonzero(free)[0], 3).astype(np.int64), coords,
+                          a.indptr, a.indices, gpu, np.zeros((ndof, 1)))
+    h, err, _ = ffi.Handle.factor(prob, precision="mixed")
+    if h is not None and err == -1:
+        tr, th = [], []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            a.refactor(h, young, nu)
+            torch.cuda.synchronize()
+            tr.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            h.refactor(a.values(young, nu))
+            torch.cuda.synchronize()
+            th.append(time.perf_counter() - t0)
+        res["refactor_from_assembly_s"] = min(tr)
+        res["refactor_host_round_trip_s"] = min(th)
+        h.free()
+    log(f"assembly {n}^3: step {step_s * 1e3:.2f} ms (element {kms:.3f} ms, gather {gms:.3f} ms), "
+        f"oracle {cpu_s:.2f} s, mismatches {mismatches}")
+    return res
+
+
+def run_assembly(args):
+    print(json.dumps(assembly_result(args)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -591,7 +694,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="direct workload only: skip the batch and learned-smoother sub-benchmarks")
     ap.add_argument("--indef-size", type=int, default=80, help="grid edge of the shifted Laplacian (indef workload)")
-    ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned", "ic0", "indef"], default="direct",
+    ap.add_argument("--workload", choices=["direct", "amg", "batch", "learned", "ic0", "indef", "assembly"],
+                    default="direct",
                     help="direct: the multifrontal block (default); amg: the heterogeneous block with AMG-Krylov; "
                          "batch: 512 plates factored and solved together; learned: the C3D4 block with the "
                          "learned-smoother FCG")
@@ -619,6 +723,9 @@ def main():
         return
     if args.workload == "indef":
         run_indef(args)
+        return
+    if args.workload == "assembly":
+        run_assembly(args)
         return
     os.environ.setdefault("RUSTYFEM_KERNEL_TIMING", "1")
     import numpy as np
@@ -727,7 +834,8 @@ def main():
     if not args.no_extras:
         # the other configurations, each with its own roofline, CPU baseline
         # and error against the FP64 oracle; a failure is recorded, not raised
-        for key, fn in (("c3_batch", batch_result), ("c5_learned", learned_result)):
+        for key, fn in (("c3_batch", batch_result), ("c5_learned", learned_result),
+                        ("f1_assembly", assembly_result)):
             t0 = time.perf_counter()
             try:
                 r = fn(args)

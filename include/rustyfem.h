@@ -370,6 +370,58 @@ int32_t rustyfem_check_plan(size_t n, size_t nn, const int64_t *loc, const doubl
                             const int64_t *indices, const double *data, int32_t leaf, int32_t plant, char *msg,
                             size_t msg_len);
 
+/* ------------------------------------------------------------ assembly */
+/* Element assembly of linear elastic stiffness matrices on the GPU
+ * (additive): replaces the element loop and the serial SystemMatrix::add
+ * scatter of src/fem/assembly.rs:315-401 (stiffness() of each element,
+ * symmetric part 0.5 (Ke + Ke^T), zero terms skipped) for
+ *   RUSTYFEM_C3D8 (1) hexahedron, 2x2x2 Gauss, volumetric B-bar
+ *                     (src/elements/continuum3d.rs:39-121),
+ *   RUSTYFEM_C3D4 (2) tetrahedron, one point (src/elements/solid_higher.rs:109),
+ *   RUSTYFEM_CPS4 (3) quadrilateral, plane stress, 2x2 Gauss,
+ *   RUSTYFEM_CPE4 (4) quadrilateral, plane strain, planar B-bar
+ *                     (src/elements/continuum2d.rs:24-27),
+ * isotropic elastic material per element (src/materials/elastic.rs:55-75).
+ * The values equal the reference's sequential evaluation bit for bit. */
+#define RUSTYFEM_C3D8 1
+#define RUSTYFEM_C3D4 2
+#define RUSTYFEM_CPS4 3
+#define RUSTYFEM_CPE4 4
+typedef struct AssemblyHandle AssemblyHandle;
+
+/* The pattern and the per-entry contribution lists of a mesh (host work, done
+ * once): conn nelem x nodes per element (8 or 4) node indices, coords nn x 3
+ * (plane elements use x, y), eq nn x DOFs per node (3 or 2) equation numbers
+ * in 0..n-1, -1 for a DOF without equation (constrained); every equation
+ * belongs to one DOF.  NULL on invalid input (rustyfem_last_error). */
+AssemblyHandle *rustyfem_assembly_create(int32_t etype, size_t nelem, const int64_t *conn, size_t nn,
+                                         const double *coords, const int64_t *eq, size_t n);
+/* stored entries of the assembled matrix (CSC, both triangles) */
+int64_t rustyfem_assembly_nnz(const AssemblyHandle *a);
+/* indptr (n + 1) and indices (nnz) of that CSC, rows sorted per column */
+void rustyfem_assembly_pattern(const AssemblyHandle *a, int64_t *indptr, int64_t *indices);
+/* Values (nnz, on the host, or on the device with on_device = 1) for Young's
+ * modulus young[e], Poisson's ratio nu[e] and thickness[e] (plane elements;
+ * NULL = 1) of every element.  -1 on success, the lowest element whose
+ * Jacobian determinant fails the reference's test (an element that is
+ * "distorted or inverted", src/elements/common.rs:15), or -2 on an error. */
+int64_t rustyfem_assembly_values(AssemblyHandle *a, const double *young, const double *nu, const double *thickness,
+                                 double *data, int32_t on_device);
+/* Assembles straight into the factorization's own value layout (the lower
+ * triangle in its nested-dissection numbering, on the device) and refactors
+ * h, whose pattern must be this assembly's (a positive definite handle from
+ * rustyfem_factor or rustyfem_factor_ex with rustyfem_assembly_pattern's
+ * indptr/indices and every DOF coupled): the reference's zero / add / factor
+ * cycle without a host copy of the matrix.  0 on success, 1 on a failing pivot
+ * (*err_dof, caller numbering), 2 on a distorted element (*err_elem), -1 on
+ * an error. */
+int32_t rustyfem_assembly_refactor(AssemblyHandle *a, Handle *h, const double *young, const double *nu,
+                                   const double *thickness, int64_t *err_dof, int64_t *err_elem);
+/* [element kernel ms, gather kernel ms, host setup s, contributions,
+ * doubles per stored element matrix, bound-layout gather ms]; at most 6 */
+void rustyfem_assembly_stats(const AssemblyHandle *a, double *out, size_t len);
+void rustyfem_assembly_free(AssemblyHandle *a);
+
 #ifdef __cplusplus
 }
 #endif

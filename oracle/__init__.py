@@ -99,6 +99,9 @@ def lib():
             "ora_eigen": (C.c_int64, [vp, sz, i64p, i64p, f64p, sz, sz, sz, C.c_double, f64p, vp,
                                       C.POINTER(C.c_int64)]),
             "ora_jacobi_eigen": (None, [sz, f64p, f64p, f64p]),
+            "ora_element_stiffness": (C.c_int32, [C.c_int32, f64p, C.c_double, C.c_double, C.c_double, f64p]),
+            "ora_assemble": (C.c_int64, [C.c_int32, C.c_int64, i64p, C.c_int64, f64p, i64p, C.c_int64, f64p, f64p,
+                                         vp, i64p, i64p, f64p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -450,6 +453,49 @@ def jacobi_eigen(a):
     v = np.zeros(max(n * n, 1))
     lib().ora_jacobi_eigen(n, a.reshape(-1), d, v)
     return d[:n], v[:n * n].reshape(n, n)
+
+
+ELEMENT_TYPES = {"C3D8": 1, "C3D4": 2, "CPS4": 3, "CPE4": 4}
+
+
+def element_stiffness(etype, x, young, nu, thickness=1.0):
+    """The oracle's element stiffness (nd x nd) of a C3D8, C3D4, CPS4 or CPE4
+    with node coordinates x (nodes x 3); None for a distorted element."""
+    code = ELEMENT_TYPES[etype]
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    nodes = len(x) // 3
+    nd = nodes * (3 if code <= 2 else 2)
+    ke = np.zeros(nd * nd)
+    r = lib().ora_element_stiffness(code, x, float(young), float(nu), float(thickness), ke)
+    if r == -1:
+        return None
+    if r != 0:
+        raise ValueError(f"ora_element_stiffness failed ({r})")
+    return ke.reshape(nd, nd)
+
+
+def assemble(etype, conn, coords, eq, n, young, nu, indptr, indices, thickness=None):
+    """The reference's element loop and scatter into the given both-triangle
+    CSC pattern: the data array, or the index of the first distorted element
+    (an int)."""
+    code = ELEMENT_TYPES[etype]
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    eq = np.ascontiguousarray(eq, dtype=np.int64)
+    young = np.ascontiguousarray(young, dtype=np.float64)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    th = None if thickness is None else np.ascontiguousarray(thickness, dtype=np.float64)
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.int64)
+    data = np.zeros(max(int(indptr[-1]), 1))
+    r = lib().ora_assemble(code, conn.shape[0], conn.reshape(-1), coords.shape[0], coords.reshape(-1),
+                           eq.reshape(-1), int(n), young, nu, None if th is None else th.ctypes.data, indptr,
+                           indices, data)
+    if r >= 0:
+        return int(r)
+    if r != -1:
+        raise ValueError(f"ora_assemble failed ({r})")
+    return data[:int(indptr[-1])]
 
 
 def solve_lower(n, colptr, rowidx, val, b, kind="amg", coords=None, ndof=None, tol=1e-10, maxit=0):

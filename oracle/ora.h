@@ -125,6 +125,18 @@ int64_t ora_eigen(const ora_handle *h, size_t n, const int64_t *bp, const int64_
  * component i of eigenvector j */
 void ora_jacobi_eigen(size_t n, const double *a, double *d, double *v);
 
+/* ---- element stiffness and assembly (ora_elem.c) ---- */
+/* etype 1 C3D8 (B-bar), 2 C3D4, 3 CPS4, 4 CPE4; x: nodes x 3; ke: nd x nd
+ * row-major (src/elements/).  0, -1 (distorted), -2 (unknown type) */
+int32_t ora_element_stiffness(int32_t etype, const double *x, double young, double nu, double thickness,
+                              double *ke);
+/* The element loop and scatter of src/fem/assembly.rs:378-398 into a given
+ * both-triangle CSC pattern (eq: nn x dofs per node, -1 = no equation).
+ * Returns -1, the first distorted element, -2 (entry missing) or -3. */
+int64_t ora_assemble(int32_t etype, int64_t nelem, const int64_t *conn, int64_t nn, const double *coords,
+                     const int64_t *eq, int64_t n, const double *young, const double *nu, const double *thickness,
+                     const int64_t *indptr, const int64_t *indices, double *data);
+
 /* ---- iterative path (src/solver/iterative*.rs) ---- */
 /* kind: 0 AMG, 1 IC(0), 2 Jacobi.  Returns iterations; rel (relative recursive residual), converged. */
 int64_t ora_solve_lower(size_t n, const int64_t *colptr, const int64_t *rowidx, const double *val,

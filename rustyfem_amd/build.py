@@ -19,9 +19,10 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HOST_SOURCES = ["symbolic.cpp", "mf_plan.cpp", "plan_check.cpp", "amg_setup.cpp", "ffi.cpp"]
-HIP_SOURCES = ["mf_gpu.hip", "indef_gpu.hip", "pcg_gpu.hip", "mpnn_gpu.hip", "ic0_gpu.hip", "eigen_gpu.hip"]
+HIP_SOURCES = ["mf_gpu.hip", "indef_gpu.hip", "pcg_gpu.hip", "mpnn_gpu.hip", "ic0_gpu.hip", "eigen_gpu.hip",
+               "assembly_gpu.hip"]
 # kernels that restate a CPU operation order entry by entry: no contraction
-NO_CONTRACT = {"indef_gpu.hip"}
+NO_CONTRACT = {"indef_gpu.hip", "assembly_gpu.hip"}
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",

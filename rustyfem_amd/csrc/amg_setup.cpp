@@ -1,7 +1,10 @@
 // Host setup of the smoothed-aggregation AMG hierarchy (restates
 // src/solver/iterative/amg.rs and the CSR kernels of csr.rs).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <stdexcept>
 #include <thread>
@@ -409,21 +412,33 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
   const double fine_nnz = static_cast<double>(a.nnz());
   double total = fine_nnz;
   double theta = 0.08;
+  // RUSTYFEM_AMG_TIMING=1: the seconds of each setup phase per level, on stderr
+  const char* tenv = std::getenv("RUSTYFEM_AMG_TIMING");
+  const bool timing = tenv && *tenv && *tenv != '0';
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto secs = [](std::chrono::steady_clock::time_point a0, std::chrono::steady_clock::time_point a1) {
+    return std::chrono::duration<double>(a1 - a0).count();
+  };
   while (a.nrows > 1500 && H.levels.size() < 12) {
+    const auto t0 = now();
     std::vector<double> diag = csr_diag(a);
     for (double d : diag)
       if (!(d > 0.0)) throw AmgBreakdown("AMG setup: a diagonal entry is not positive");
     int64_t nagg = 0;
     const std::vector<int64_t> agg = amg_aggregate(a, bptr, theta, nagg);
+    const auto t1 = now();
     std::vector<double> bc;
     std::vector<int64_t> cptr;
     Csr pt = tentative(bptr, agg, nagg, b, nb, bc, cptr);
+    const auto t2 = now();
     if (static_cast<double>(pt.ncols) > 0.85 * static_cast<double>(a.nrows)) break;
     const double rho = spectral_radius(a, diag);
     const double omega = 4.0 / 3.0 / rho;
+    const auto t3 = now();
     Csr ap = csr_mul(a, pt);
     Csr p = sub_scaled(pt, ap, diag, omega);
     ap = Csr();
+    const auto t4 = now();
     AmgHostLevel L;
     if (keep) {
       L.agg = agg;
@@ -433,7 +448,18 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
     }
     pt = Csr();
     Csr r = csr_transpose(p);
-    Csr ac = csr_mul(r, csr_mul(a, p));
+    const auto t5 = now();
+    Csr apm = csr_mul(a, p);
+    const auto t6 = now();
+    Csr ac = csr_mul(r, apm);
+    apm = Csr();
+    const auto t7 = now();
+    if (timing)
+      std::fprintf(stderr,
+                   "[amg] level %zu: n %lld nnz %lld | aggregate %.3f s, tentative %.3f, spectral radius %.3f, "
+                   "A P~ and smoothing %.3f, transpose %.3f, A P %.3f, R (A P) %.3f\n",
+                   H.levels.size(), static_cast<long long>(a.nrows), static_cast<long long>(a.nnz()), secs(t0, t1),
+                   secs(t1, t2), secs(t2, t3), secs(t3, t4), secs(t4, t5), secs(t5, t6), secs(t6, t7));
     total += static_cast<double>(ac.nnz());
     H.sizes.push_back(ac.nrows);
     L.nblk = static_cast<int64_t>(bptr.size()) - 1;

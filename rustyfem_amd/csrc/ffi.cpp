@@ -5,6 +5,7 @@
 // entry filter per mode); the numeric factorization and the solves run on
 // the GPU (mf_gpu.hip, indef_gpu.hip).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +19,7 @@
 
 #include "../../include/rustyfem.h"
 #include "amg.hpp"
+#include "assembly.hpp"
 #include "eigen.hpp"
 #include "gpu.hpp"
 #include "indef.hpp"
@@ -48,9 +50,16 @@ Options resolve(const rustyfem_options* o) {
   return r;
 }
 
+// a number per handle that the process never gives twice
+uint64_t next_serial() {
+  static std::atomic<uint64_t> s{0};
+  return ++s;
+}
+
 }  // namespace
 
 struct Handle {
+  uint64_t serial = next_serial();
   Mode mode = Mode::Pd;
   Options opt;
   int64_t n = 0;                 // caller DOFs
@@ -1211,5 +1220,89 @@ int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t* loc, const double* 
   }
   return -1;
 }
+
+struct AssemblyHandle {
+  std::unique_ptr<rf::GpuAssembly> a;
+};
+
+AssemblyHandle* rustyfem_assembly_create(int32_t etype, size_t nelem, const int64_t* conn, size_t nn,
+                                         const double* coords, const int64_t* eq, size_t n) {
+  set_error("");
+  try {
+    if (etype < RUSTYFEM_C3D8 || etype > RUSTYFEM_CPE4) throw std::runtime_error("unknown element type");
+    auto h = std::make_unique<AssemblyHandle>();
+    h->a = rf::make_gpu_assembly(static_cast<int64_t>(nelem), static_cast<rf::ElemType>(etype), conn,
+                                 static_cast<int64_t>(nn), coords, eq, static_cast<int64_t>(n));
+    return h.release();
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return nullptr;
+}
+
+int64_t rustyfem_assembly_nnz(const AssemblyHandle* a) { return a->a->nnz(); }
+
+void rustyfem_assembly_pattern(const AssemblyHandle* a, int64_t* indptr, int64_t* indices) {
+  const auto& p = a->a->indptr();
+  const auto& i = a->a->indices();
+  std::copy(p.begin(), p.end(), indptr);
+  std::copy(i.begin(), i.end(), indices);
+}
+
+int64_t rustyfem_assembly_values(AssemblyHandle* a, const double* young, const double* nu, const double* thickness,
+                                 double* data, int32_t on_device) {
+  set_error("");
+  try {
+    return a->a->values(young, nu, thickness, data, on_device != 0);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -2;
+}
+
+int32_t rustyfem_assembly_refactor(AssemblyHandle* a, Handle* h, const double* young, const double* nu,
+                                   const double* thickness, int64_t* err_dof, int64_t* err_elem) {
+  *err_dof = -1;
+  *err_elem = -1;
+  set_error("");
+  try {
+    if (h->mode != Mode::Pd || !h->gpu) throw std::runtime_error("assembly refactor needs a positive definite handle");
+    if (a->a->bound_owner() != h->serial) {
+      if (h->indptr != a->a->indptr() || h->indices != a->a->indices())
+        throw std::runtime_error("the handle was factored on another pattern than the assembly's");
+      for (int64_t i = 0; i < h->n; ++i)
+        if (!h->coupled[i]) throw std::runtime_error("the handle has an uncoupled DOF");
+      a->a->bind(h->vmap, static_cast<int64_t>(h->rowidx.size()), h->serial);
+    }
+    const int64_t bad = a->a->values_bound(young, nu, thickness);
+    if (bad >= 0) {
+      *err_elem = bad;
+      return 2;
+    }
+    // the numeric step of a positive definite refactorization from device values
+    h->gpu64.reset();
+    const int64_t e = h->gpu->factor(a->a->bound_values(), true);
+    if (e >= 0) {
+      *err_dof = h->owner[e];
+      return 1;
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+  } catch (...) {
+    set_error("unknown error");
+  }
+  return -1;
+}
+
+void rustyfem_assembly_stats(const AssemblyHandle* a, double* out, size_t len) {
+  a->a->stats(out, static_cast<int>(std::min<size_t>(len, 6)));
+}
+
+void rustyfem_assembly_free(AssemblyHandle* a) { delete a; }
 
 }  // extern "C"

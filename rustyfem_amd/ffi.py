@@ -26,7 +26,9 @@ EXPORTS = ["rustyfem_factor", "rustyfem_factor_indef", "rustyfem_factor_unsym", 
            "rustyfem_batch_info", "rustyfem_batch_stats", "rustyfem_batch_order", "rustyfem_free_batch",
            "rustyfem_pcg_set_learned_smoother", "rustyfem_mpnn_nparams", "rustyfem_mpnn_create", "rustyfem_mpnn_loss",
            "rustyfem_mpnn_smoother", "rustyfem_mpnn_free", "rustyfem_pcg_precondition", "rustyfem_chebyshev",
-           "rustyfem_pcg_setup_ex", "rustyfem_pcg_level", "rustyfem_amg_aggregate", "rustyfem_eigen"]
+           "rustyfem_pcg_setup_ex", "rustyfem_pcg_level", "rustyfem_amg_aggregate", "rustyfem_eigen",
+           "rustyfem_assembly_create", "rustyfem_assembly_nnz", "rustyfem_assembly_pattern", "rustyfem_assembly_values",
+           "rustyfem_assembly_refactor", "rustyfem_assembly_stats", "rustyfem_assembly_free"]
 
 _lib = None
 
@@ -145,6 +147,20 @@ def lib():
         L.rustyfem_solve_lower.restype = C.c_int64
         L.rustyfem_solve_lower.argtypes = [sz, vp, vp, vp, vp, C.c_int32, sz, vp, vp, C.c_double, C.c_int64, vp, f64p,
                                            C.POINTER(C.c_int32)]
+        L.rustyfem_assembly_create.restype = vp
+        L.rustyfem_assembly_create.argtypes = [C.c_int32, sz, vp, sz, vp, vp, sz]
+        L.rustyfem_assembly_nnz.restype = C.c_int64
+        L.rustyfem_assembly_nnz.argtypes = [vp]
+        L.rustyfem_assembly_pattern.restype = None
+        L.rustyfem_assembly_pattern.argtypes = [vp, vp, vp]
+        L.rustyfem_assembly_values.restype = C.c_int64
+        L.rustyfem_assembly_values.argtypes = [vp, vp, vp, vp, vp, C.c_int32]
+        L.rustyfem_assembly_refactor.restype = C.c_int32
+        L.rustyfem_assembly_refactor.argtypes = [vp, vp, vp, vp, vp, i64p, i64p]
+        L.rustyfem_assembly_stats.restype = None
+        L.rustyfem_assembly_stats.argtypes = [vp, vp, sz]
+        L.rustyfem_assembly_free.restype = None
+        L.rustyfem_assembly_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -599,3 +615,86 @@ def amg_aggregate(a, bptr, theta):
     if n < 0:
         raise RuntimeError(last_error())
     return agg, int(n)
+
+
+ELEMENT_TYPES = {"C3D8": 1, "C3D4": 2, "CPS4": 3, "CPE4": 4}
+
+
+class DistortedElement(ValueError):
+    """An element whose Jacobian determinant fails the reference's test
+    (src/elements/common.rs:15); .element is its index."""
+
+    def __init__(self, element):
+        super().__init__(f"element {element} is distorted or inverted (negative Jacobian)")
+        self.element = element
+
+
+class Assembly:
+    """GPU element assembly of a mesh (rustyfem_assembly_*): the pattern once,
+    then the stiffness values for any per-element material."""
+
+    def __init__(self, etype, conn, coords, eq, n):
+        code = ELEMENT_TYPES[etype]
+        self.etype = etype
+        self._conn = np.ascontiguousarray(conn, dtype=np.int64)
+        self._coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self._eq = np.ascontiguousarray(eq, dtype=np.int64)
+        self.nelem = self._conn.shape[0]
+        self.n = int(n)
+        self._h = lib().rustyfem_assembly_create(code, self.nelem, self._conn.ctypes.data, self._coords.shape[0],
+                                                 self._coords.ctypes.data, self._eq.ctypes.data, self.n)
+        if not self._h:
+            raise RuntimeError("rustyfem_assembly_create failed: " + last_error())
+        self.nnz = int(lib().rustyfem_assembly_nnz(self._h))
+        self.indptr = np.zeros(self.n + 1, dtype=np.int64)
+        self.indices = np.zeros(max(self.nnz, 1), dtype=np.int64)
+        lib().rustyfem_assembly_pattern(self._h, self.indptr.ctypes.data, self.indices.ctypes.data)
+        self.indices = self.indices[:self.nnz]
+
+    def _material(self, young, nu, thickness):
+        y = np.ascontiguousarray(np.broadcast_to(np.asarray(young, dtype=np.float64), (self.nelem,)))
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(nu, dtype=np.float64), (self.nelem,)))
+        t = None if thickness is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(thickness, dtype=np.float64), (self.nelem,)))
+        return y, v, t
+
+    def values(self, young, nu, thickness=None, out=None):
+        """The nnz values (host array), or into a device tensor `out`."""
+        y, v, t = self._material(young, nu, thickness)
+        tp = None if t is None else t.ctypes.data
+        if out is not None:
+            r = lib().rustyfem_assembly_values(self._h, y.ctypes.data, v.ctypes.data, tp, out.data_ptr(), 1)
+            data = out
+        else:
+            data = np.zeros(max(self.nnz, 1))
+            r = lib().rustyfem_assembly_values(self._h, y.ctypes.data, v.ctypes.data, tp, data.ctypes.data, 0)
+            data = data[:self.nnz]
+        if r == -2:
+            raise RuntimeError("rustyfem_assembly_values failed: " + last_error())
+        if r >= 0:
+            raise DistortedElement(int(r))
+        return data
+
+    def refactor(self, handle, young, nu, thickness=None):
+        """Assemble into handle's factorization layout and refactor it: -1, or
+        the caller DOF of a failing pivot."""
+        y, v, t = self._material(young, nu, thickness)
+        ed, ee = C.c_int64(0), C.c_int64(0)
+        rc = lib().rustyfem_assembly_refactor(self._h, handle._h, y.ctypes.data, v.ctypes.data,
+                                              None if t is None else t.ctypes.data, C.byref(ed), C.byref(ee))
+        if rc < 0:
+            raise RuntimeError("rustyfem_assembly_refactor failed: " + last_error())
+        if rc == 2:
+            raise DistortedElement(int(ee.value))
+        return int(ed.value) if rc == 1 else -1
+
+    def stats(self):
+        o = np.zeros(6)
+        lib().rustyfem_assembly_stats(self._h, o.ctypes.data, 6)
+        return {"element_ms": o[0], "gather_ms": o[1], "setup_s": o[2], "contributions": int(o[3]),
+                "doubles_per_element": int(o[4]), "bound_gather_ms": o[5]}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().rustyfem_assembly_free(self._h)
+            self._h = None

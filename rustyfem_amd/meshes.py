@@ -352,6 +352,37 @@ def hex_block(nx, ny=None, nz=None, e=210e9, nu=0.3, escale=None, clamp="x0", lo
                    meta={"nx": nx, "ny": ny, "nz": nz, "e": e, "nu": nu, "clamp": clamp, "load": load})
 
 
+def grid_elements(shape_e):
+    """Connectivity of the structured grid's elements (the box and plate
+    meshers' node numbering, x fastest): (prod(shape_e), 8) for three axes in
+    the C3D8 corner order (HEX8_NAT), (prod(shape_e), 4) for two axes in the
+    CPS4/CPE4 order (counter-clockwise from (0, 0)); elements x fastest."""
+    shape_n = [s + 1 for s in shape_e]
+    if len(shape_e) == 2:
+        nx, ny = shape_e
+        jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        base = (ii + shape_n[0] * jj).ravel()
+        corner = np.array([0, 1, 1 + shape_n[0], shape_n[0]])
+    else:
+        nx, ny, nz = shape_e
+        kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        base = (ii + shape_n[0] * (jj + shape_n[1] * kk)).ravel()
+        nat = (HEX8_NAT.astype(np.int64) + 1) // 2
+        corner = nat[:, 0] + shape_n[0] * (nat[:, 1] + shape_n[1] * nat[:, 2])
+    return (base[:, None] + corner[None, :]).astype(np.int64)
+
+
+def equation_numbers(free, dim):
+    """(nn, dim) equation numbers of the free nodes, numbered node-major
+    (node order, then component) as the problems here number their DOFs; -1
+    for the DOFs of fixed nodes."""
+    free = np.asarray(free, dtype=bool).ravel()
+    eq = np.full((free.size, dim), -1, dtype=np.int64)
+    nf = int(free.sum())
+    eq[free] = np.arange(nf * dim, dtype=np.int64).reshape(nf, dim)
+    return eq
+
+
 def plate_cpe4(nx, ny, e, nu, nrhs=8, seed=3, thickness=1.0):
     """A CPE4 (planar B-bar) plate of unit height: h = 1/ny, length nx*h, clamped at
     x=0, with `nrhs` random edge-traction load cases on the other three edges."""
@@ -471,7 +502,7 @@ def tet4_stiffness(x: np.ndarray, e: np.ndarray, nu: float):
 
 
 def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4, perturb=0.2, chunk=400000,
-              clamp="z0", min_quality=0.1):
+              clamp="z0", min_quality=0.1, elements=False):
     """C3D4 block on nx^3 cubes of the unit cube, each split into six tetrahedra
     (KUHN), interior vertices moved by U[-perturb h, perturb h]^3 (the moves of
     the vertices of a tetrahedron left with less than min_quality of its
@@ -479,7 +510,9 @@ def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4
     Young's modulus per cube (log-std drawn from U[0.5, 2] when not given,
     correlation length corr_len), clamped base z = 0 (clamp=None: no support),
     body load along -z.
-    Nodes are numbered as the box mesher numbers them (x fastest)."""
+    Nodes are numbered as the box mesher numbers them (x fastest).  With
+    elements=True the connectivity and per-element Young's moduli are kept in
+    meta["conn"], meta["young"] and the equation numbers in meta["eq"]."""
     rng = np.random.default_rng(seed)
     if log_std is None:
         log_std = float(rng.uniform(0.5, 2.0))
@@ -571,8 +604,10 @@ def tet_block(nx, log_std=None, corr_len=0.1, seed=42, e=210e9, nu=0.3, body=1e4
     loc = np.repeat(np.nonzero(free)[0], 3).astype(np.int64)
     rhs = np.zeros((n, 1), order="F")
     rhs[:, 0] = f[free].ravel()
-    return Problem(f"tet_c3d4_{nx}", n, nn, loc, coords, indptr, indices, data, rhs, ndof=3,
-                   meta={"nx": nx, "log_std": log_std, "seed": seed, "e": e, "nu": nu})
+    meta = {"nx": nx, "log_std": log_std, "seed": seed, "e": e, "nu": nu}
+    if elements:
+        meta.update(conn=conn.astype(np.int64), young=estiff, eq=equation_numbers(free, 3))
+    return Problem(f"tet_c3d4_{nx}", n, nn, loc, coords, indptr, indices, data, rhs, ndof=3, meta=meta)
 
 
 def _stencil_csc(sten, offs, shape_n, free):
