@@ -589,9 +589,8 @@ def assembly_result(args):
     fed straight from the assembly (rustyfem_assembly_refactor) against the
     host round trip (values to the host, rustyfem_refactor)."""
     import numpy as np
-    import torch
     import oracle
-    from rustyfem_amd import ffi, meshes
+    from rustyfem_amd import ffi, hip, meshes
     ffi.lib()
     n = args.size
     kk, jj, ii = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
@@ -605,21 +604,22 @@ def assembly_result(args):
     t0 = time.perf_counter()
     a = ffi.Assembly("C3D8", conn, coords, eq, ndof)
     setup = time.perf_counter() - t0
-    out_t = torch.empty(a.nnz, dtype=torch.float64, device="cuda")
+    out_d = hip.DeviceArray(a.nnz)
     steps = max(args.steps, 3)
     for _ in range(max(args.warmup, 1)):
-        a.values(young, nu, out=out_t)
-    torch.cuda.synchronize()
+        a.values(young, nu, out=out_d.ptr)
+    hip.synchronize()
     ke_ms, ga_ms = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        a.values(young, nu, out=out_t)
+        a.values(young, nu, out=out_d.ptr)
         st = a.stats()
         ke_ms.append(st["element_ms"])
         ga_ms.append(st["gather_ms"])
-    torch.cuda.synchronize()
+    hip.synchronize()
     step_s = (time.perf_counter() - t0) / steps
-    gpu = out_t.cpu().numpy()
+    gpu = out_d.download()
+    out_d.free()
     t0 = time.perf_counter()
     ref = oracle.assemble("C3D8", conn, coords, eq, ndof, young, nu, a.indptr, a.indices)
     cpu_s = time.perf_counter() - t0
@@ -658,14 +658,14 @@ def assembly_result(args):
     if h is not None and err == -1:
         tr, th = [], []
         for _ in range(2):
-            torch.cuda.synchronize()
+            hip.synchronize()
             t0 = time.perf_counter()
             a.refactor(h, young, nu)
-            torch.cuda.synchronize()
+            hip.synchronize()
             tr.append(time.perf_counter() - t0)
             t0 = time.perf_counter()
             h.refactor(a.values(young, nu))
-            torch.cuda.synchronize()
+            hip.synchronize()
             th.append(time.perf_counter() - t0)
         res["refactor_from_assembly_s"] = min(tr)
         res["refactor_host_round_trip_s"] = min(th)

@@ -659,11 +659,13 @@ class Assembly:
         return y, v, t
 
     def values(self, young, nu, thickness=None, out=None):
-        """The nnz values (host array), or into a device tensor `out`."""
+        """The nnz values (host array), or into device memory at the address
+        `out` (nnz doubles)."""
         y, v, t = self._material(young, nu, thickness)
         tp = None if t is None else t.ctypes.data
         if out is not None:
-            r = lib().rustyfem_assembly_values(self._h, y.ctypes.data, v.ctypes.data, tp, out.data_ptr(), 1)
+            addr = out.value if isinstance(out, C.c_void_p) else int(out)
+            r = lib().rustyfem_assembly_values(self._h, y.ctypes.data, v.ctypes.data, tp, addr, 1)
             data = out
         else:
             data = np.zeros(max(self.nnz, 1))
