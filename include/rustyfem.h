@@ -265,7 +265,10 @@ int64_t rustyfem_pcg_solve(PcgHandle *h, const double *b, double *x, double tol,
  * 1 set up because a diagonal entry is not positive, 2 because the AMG
  * hierarchy broke down, 3 built when a solve did not converge, [41] 1 when the
  * last solve's result came from the direct solver, [42] the true relative
- * residual ||b - A x|| / ||b|| of that direct solution (-1 otherwise). */
+ * residual ||b - A x|| / ||b|| of that direct solution (-1 otherwise), [43] 1
+ * when the AMG hierarchy was built on the GPU (0: on the host, as with
+ * RUSTYFEM_AMG_HOST_SETUP=1 or a matrix row too long for the device setup's
+ * LDS tables).  At most 44 entries. */
 void rustyfem_pcg_stats(const PcgHandle *h, double *out, size_t len);
 
 /* Host copy of AMG level `level` of a handle set up with

@@ -81,4 +81,14 @@ std::vector<double> coarse_inverse(const std::vector<double>& l, const std::vect
 // aggregation of one level (src/solver/iterative/amg.rs:108-181); exposed for tests
 std::vector<int64_t> amg_aggregate(const Csr& a, const std::vector<int64_t>& bptr, double theta, int64_t& nagg);
 
+// the three passes of the aggregation (src/solver/iterative/amg.rs:140-181)
+// over the strong neighbours of every block (CSR sptr/sidx, each list in the
+// order of first appearance in the block's rows)
+std::vector<int64_t> aggregate_passes(int64_t nb, const std::vector<int64_t>& sptr, const std::vector<int32_t>& sidx,
+                                      int64_t& nagg);
+
+// dense LDL^T of the coarsest matrix (src/solver/iterative/amg.rs:319-340):
+// unit L row-major and D; throws AmgBreakdown when a pivot d <= 1e-13 max|a_ii|
+void coarse_ldlt(const Csr& a, std::vector<double>& l, std::vector<double>& d);
+
 }  // namespace rf

@@ -205,6 +205,46 @@ std::vector<double> rigid_body_modes(int64_t nn, const double* coords, const int
   return b;
 }
 
+std::vector<int64_t> aggregate_passes(int64_t nb, const std::vector<int64_t>& sptr, const std::vector<int32_t>& sidx,
+                                      int64_t& nagg) {
+  std::vector<int64_t> agg(nb, -1);
+  int64_t n = 0;
+  // pass 1: blocks whose strong neighbours are all free start an aggregate
+  for (int64_t b = 0; b < nb; ++b) {
+    if (agg[b] != -1) continue;
+    bool all = true;
+    for (int64_t q = sptr[b]; q < sptr[b + 1]; ++q)
+      if (agg[sidx[q]] != -1) {
+        all = false;
+        break;
+      }
+    if (!all) continue;
+    agg[b] = n;
+    for (int64_t q = sptr[b]; q < sptr[b + 1]; ++q) agg[sidx[q]] = n;
+    ++n;
+  }
+  // pass 2: join the aggregate of the first strong neighbour (as of pass 1)
+  const std::vector<int64_t> snap = agg;
+  for (int64_t b = 0; b < nb; ++b) {
+    if (agg[b] != -1) continue;
+    for (int64_t q = sptr[b]; q < sptr[b + 1]; ++q)
+      if (snap[sidx[q]] != -1) {
+        agg[b] = snap[sidx[q]];
+        break;
+      }
+  }
+  // pass 3: the rest form aggregates with their free strong neighbours
+  for (int64_t b = 0; b < nb; ++b) {
+    if (agg[b] != -1) continue;
+    agg[b] = n;
+    for (int64_t q = sptr[b]; q < sptr[b + 1]; ++q)
+      if (agg[sidx[q]] == -1) agg[sidx[q]] = n;
+    ++n;
+  }
+  nagg = n;
+  return agg;
+}
+
 std::vector<int64_t> amg_aggregate(const Csr& a, const std::vector<int64_t>& bptr, double theta, int64_t& nagg) {
   const int64_t nb = static_cast<int64_t>(bptr.size()) - 1;
   std::vector<int64_t> blk(bptr[nb]);
@@ -215,7 +255,9 @@ std::vector<int64_t> amg_aggregate(const Csr& a, const std::vector<int64_t>& bpt
   for (int64_t i = 0; i < a.nrows; ++i)
     for (int64_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p)
       if (blk[a.idx[p]] == blk[i]) dnorm[blk[i]] += a.val[p] * a.val[p];
-  std::vector<std::vector<int64_t>> strong(nb);
+  // strong neighbours of every block, in the order of first appearance
+  std::vector<int64_t> sptr(nb + 1, 0);
+  std::vector<int32_t> sidx;
   {
     std::vector<double> acc(nb, 0.0);
     std::vector<int64_t> mark(nb, -1), list;
@@ -233,45 +275,37 @@ std::vector<int64_t> amg_aggregate(const Csr& a, const std::vector<int64_t>& bpt
           acc[c] += a.val[p] * a.val[p];
         }
       for (int64_t c : list)
-        if (acc[c] > theta * theta * std::sqrt(dnorm[b] * dnorm[c])) strong[b].push_back(c);
+        if (acc[c] > theta * theta * std::sqrt(dnorm[b] * dnorm[c])) sidx.push_back(static_cast<int32_t>(c));
+      sptr[b + 1] = static_cast<int64_t>(sidx.size());
     }
   }
-  std::vector<int64_t> agg(nb, -1);
-  int64_t n = 0;
-  // pass 1: blocks whose strong neighbours are all free start an aggregate
-  for (int64_t b = 0; b < nb; ++b) {
-    if (agg[b] != -1) continue;
-    bool all = true;
-    for (int64_t c : strong[b])
-      if (agg[c] != -1) {
-        all = false;
-        break;
-      }
-    if (!all) continue;
-    agg[b] = n;
-    for (int64_t c : strong[b]) agg[c] = n;
-    ++n;
+  return aggregate_passes(nb, sptr, sidx, nagg);
+}
+
+void coarse_ldlt(const Csr& a, std::vector<double>& l, std::vector<double>& d) {
+  // dense LDL^T of the coarsest matrix (amg.rs:319-340)
+  const int64_t n = a.nrows;
+  l.assign(static_cast<size_t>(n) * n, 0.0);
+  d.assign(n, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) l[i * n + a.idx[p]] = a.val[p];
+  double dmax = 0.0;
+  for (int64_t i = 0; i < n; ++i) dmax = std::fmax(dmax, std::fabs(l[i * n + i]));
+  for (int64_t j = 0; j < n; ++j) {
+    double dj = l[j * n + j];
+    for (int64_t k = 0; k < j; ++k) dj -= l[j * n + k] * l[j * n + k] * d[k];
+    if (!(dj > 1e-13 * dmax)) throw AmgBreakdown("AMG setup: the coarse LDL^T broke down");
+    d[j] = dj;
+    for (int64_t i = j + 1; i < n; ++i) {
+      double s = l[i * n + j];
+      for (int64_t k = 0; k < j; ++k) s -= l[i * n + k] * l[j * n + k] * d[k];
+      l[i * n + j] = s / dj;
+    }
   }
-  // pass 2: join the aggregate of the first strong neighbour (as of pass 1)
-  const std::vector<int64_t> snap = agg;
-  for (int64_t b = 0; b < nb; ++b) {
-    if (agg[b] != -1) continue;
-    for (int64_t c : strong[b])
-      if (snap[c] != -1) {
-        agg[b] = snap[c];
-        break;
-      }
+  for (int64_t i = 0; i < n; ++i) {
+    l[i * n + i] = 1.0;
+    for (int64_t j = i + 1; j < n; ++j) l[i * n + j] = 0.0;
   }
-  // pass 3: the rest form aggregates with their free strong neighbours
-  for (int64_t b = 0; b < nb; ++b) {
-    if (agg[b] != -1) continue;
-    agg[b] = n;
-    for (int64_t c : strong[b])
-      if (agg[c] == -1) agg[c] = n;
-    ++n;
-  }
-  nagg = n;
-  return agg;
 }
 
 namespace {
@@ -478,28 +512,8 @@ AmgHost amg_setup(Csr a, std::vector<int64_t> bptr, std::vector<double> b, int n
   }
   H.coarse_bptr = bptr;
   if (keep) H.coarse_B = b;
-  // dense LDL^T of the coarsest matrix (amg.rs:319-340)
-  const int64_t n = a.nrows;
-  std::vector<double> l(static_cast<size_t>(n) * n, 0.0), d(n, 0.0);
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) l[i * n + a.idx[p]] = a.val[p];
-  double dmax = 0.0;
-  for (int64_t i = 0; i < n; ++i) dmax = std::fmax(dmax, std::fabs(l[i * n + i]));
-  for (int64_t j = 0; j < n; ++j) {
-    double dj = l[j * n + j];
-    for (int64_t k = 0; k < j; ++k) dj -= l[j * n + k] * l[j * n + k] * d[k];
-    if (!(dj > 1e-13 * dmax)) throw AmgBreakdown("AMG setup: the coarse LDL^T broke down");
-    d[j] = dj;
-    for (int64_t i = j + 1; i < n; ++i) {
-      double s = l[i * n + j];
-      for (int64_t k = 0; k < j; ++k) s -= l[i * n + k] * l[j * n + k] * d[k];
-      l[i * n + j] = s / dj;
-    }
-  }
-  for (int64_t i = 0; i < n; ++i) {
-    l[i * n + i] = 1.0;
-    for (int64_t j = i + 1; j < n; ++j) l[i * n + j] = 0.0;
-  }
+  std::vector<double> l, d;
+  coarse_ldlt(a, l, d);
   H.coarse = std::move(a);
   H.coarse_l = std::move(l);
   H.coarse_d = std::move(d);

@@ -945,14 +945,15 @@ int64_t rustyfem_pcg_solve(PcgHandle* h, const double* b, double* x, double tol,
 }
 
 void rustyfem_pcg_stats(const PcgHandle* h, double* out, size_t len) {
-  double v[43];
+  double v[44];
   for (double& x : v) x = 0.0;
   if (h->solver) h->solver->stats(v, 40);
   v[25] = h->solver ? v[25] : -1.0;
   v[40] = h->direct_reason;
   v[41] = h->last_direct ? 1.0 : 0.0;
   v[42] = h->last_direct_rel;
-  for (size_t i = 0; i < len && i < 43; ++i) out[i] = v[i];
+  v[43] = h->solver && h->solver->setup_on_device() ? 1.0 : 0.0;
+  for (size_t i = 0; i < len && i < 44; ++i) out[i] = v[i];
 }
 
 int32_t rustyfem_pcg_level(const PcgHandle* h, int32_t level, int32_t what, int64_t* iout, int32_t* idx,

@@ -59,6 +59,8 @@ class PcgGpu {
   // the host AMG hierarchy (aggregates, tentative and smoothed prolongators,
   // level matrices), when the options asked to keep it; else nullptr
   virtual const AmgHost* hierarchy() const { return nullptr; }
+  // true when the AMG hierarchy was built on the GPU (else on the host)
+  virtual bool setup_on_device() const { return false; }
 };
 
 // kind: 0 AMG (smoothed aggregation, rigid body modes), 1 IC(0), 2 Jacobi.  coords /

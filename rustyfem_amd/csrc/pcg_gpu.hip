@@ -18,8 +18,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
+#include <climits>
 #include <cmath>
+#include <functional>
+#include <memory>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -684,6 +688,8 @@ void spmv_dots(const DevMat& M, const double* p, double* q, const double* r, dou
   }
 }
 
+#include "amg_device.hpp"
+
 template <typename T>
 struct VLevel {
   int64_t n = 0;
@@ -738,6 +744,16 @@ class PcgImpl final : public PcgGpu {
   template <typename T>
   void build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb);
   template <typename T>
+  void build_vcycle_dev(const DevHierarchy& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb);
+  template <typename T>
+  void init_level(VLevel<T>& V, const std::vector<double>& diag, double rho, double omega);
+  template <typename T>
+  void init_coarse(const std::vector<double>& l, const std::vector<double>& d, int64_t nc, DBuf& cm, DBuf& cx,
+                   DBuf& cb);
+  void setup_amg_host(Csr a, int64_t nn, const double* coords, const int64_t* ndof, bool f32);
+  void setup_amg_device(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int64_t nn,
+                        const double* coords, const int64_t* ndof, int bs, bool f32);
+  template <typename T>
   void vcycle(std::vector<VLevel<T>>& lv, size_t l, const int32_t* done);
   void precondition(const double* r, double* z, const int32_t* done);
   void iteration();
@@ -770,6 +786,10 @@ class PcgImpl final : public PcgGpu {
   double complexity_ = 1.0, setup_ms_ = 0, solve_ms_ = 0, spmv_ms_ = 0;
   int64_t last_it_ = 0;
   std::unique_ptr<AmgHost> kept_;  // host hierarchy, when the options keep it
+  bool device_setup_ = false;      // the hierarchy was built on the GPU (amg_device.hpp)
+
+ public:
+  bool setup_on_device() const override { return device_setup_; }
 };
 
 double PcgImpl::power_radius(const DevMat& M, const std::vector<double>& diag) {
@@ -802,6 +822,33 @@ double PcgImpl::power_radius(const DevMat& M, const std::vector<double>& diag) {
 }
 
 template <typename T>
+void PcgImpl::init_level(VLevel<T>& V, const std::vector<double>& diag, double rho, double omega) {
+  std::vector<T> di(V.n);
+  for (int64_t i = 0; i < V.n; ++i) di[i] = static_cast<T>(1.0 / diag[i]);
+  V.dinv = upload(di);
+  V.x.alloc(V.n * sizeof(T));
+  V.b.alloc(V.n * sizeof(T));
+  V.r.alloc(V.n * sizeof(T));
+  V.d.alloc(V.n * sizeof(T));
+  if (opt_.smoother == 0) {
+    chebyshev<T>(1.1 * rho, opt_.cheb_ratio, std::max(1, opt_.degree), V.c0, V.steps);
+  } else {
+    V.c0 = static_cast<T>(omega);
+    V.steps.assign(std::max(1, opt_.degree) - 1, {T(0), static_cast<T>(omega)});
+  }
+}
+
+template <typename T>
+void PcgImpl::init_coarse(const std::vector<double>& l, const std::vector<double>& d, int64_t nc, DBuf& cm, DBuf& cx,
+                          DBuf& cb) {
+  const std::vector<double> minv = coarse_inverse(l, d, nc);
+  std::vector<T> mt(minv.begin(), minv.end());
+  cm = upload(mt);
+  cx.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
+  cb.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
+}
+
+template <typename T>
 void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb) {
   const bool f32 = std::is_same<T, float>::value;
   lv.resize(H.levels.size());
@@ -815,26 +862,96 @@ void PcgImpl::build_vcycle(const AmgHost& H, std::vector<VLevel<T>>& lv, DBuf& c
     const bool ok = spmv_shape_ok(bf, bcs) && spmv_shape_ok(bcs, bf);
     V.P = make_mat(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32, true, f32 && vcycle_bf16());
     V.R = make_mat(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32, true, f32 && vcycle_bf16());
-    std::vector<T> di(V.n);
-    for (int64_t i = 0; i < V.n; ++i) di[i] = static_cast<T>(1.0 / L.diag[i]);
-    V.dinv = upload(di);
-    V.x.alloc(V.n * sizeof(T));
-    V.b.alloc(V.n * sizeof(T));
-    V.r.alloc(V.n * sizeof(T));
-    V.d.alloc(V.n * sizeof(T));
-    if (opt_.smoother == 0) {
-      chebyshev<T>(1.1 * L.rho, opt_.cheb_ratio, std::max(1, opt_.degree), V.c0, V.steps);
-    } else {
-      V.c0 = static_cast<T>(L.omega);
-      V.steps.assign(std::max(1, opt_.degree) - 1, {T(0), static_cast<T>(L.omega)});
-    }
+    init_level<T>(V, L.diag, L.rho, L.omega);
   }
-  const int64_t nc = H.coarse.nrows;
-  const std::vector<double> minv = coarse_inverse(H.coarse_l, H.coarse_d, nc);
-  std::vector<T> mt(minv.begin(), minv.end());
-  cm = upload(mt);
-  cx.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
-  cb.alloc(std::max<int64_t>(nc, 1) * sizeof(T));
+  init_coarse<T>(H.coarse_l, H.coarse_d, H.coarse.nrows, cm, cx, cb);
+}
+
+template <typename T>
+void PcgImpl::build_vcycle_dev(const DevHierarchy& H, std::vector<VLevel<T>>& lv, DBuf& cm, DBuf& cx, DBuf& cb) {
+  const bool f32 = std::is_same<T, float>::value;
+  lv.resize(H.levels.size());
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    const DevLevel& L = H.levels[l];
+    VLevel<T>& V = lv[l];
+    V.n = L.A.nrows;
+    V.A = l == 0 ? &A_ : &coarse_A_[l - 1];
+    const int bf = blk_[l], bcs = blk_[l + 1];
+    const bool ok = spmv_shape_ok(bf, bcs) && spmv_shape_ok(bcs, bf);
+    V.P = make_mat_dev(L.P, ok ? bf : 1, ok ? bcs : 1, !f32, f32, true, f32 && vcycle_bf16(), stream_);
+    V.R = make_mat_dev(L.R, ok ? bcs : 1, ok ? bf : 1, !f32, f32, true, f32 && vcycle_bf16(), stream_);
+    init_level<T>(V, L.diag, L.rho, L.omega);
+  }
+  init_coarse<T>(H.coarse_l, H.coarse_d, H.coarse.nrows, cm, cx, cb);
+}
+
+void PcgImpl::setup_amg_host(Csr a, int64_t nn, const double* coords, const int64_t* ndof, bool f32) {
+  std::vector<int64_t> bptr(nn + 1, 0);
+  for (int64_t v = 0; v < nn; ++v) bptr[v + 1] = bptr[v] + ndof[v];
+  int nb = 0;
+  std::vector<double> bm = rigid_body_modes(nn, coords, ndof, nb);
+  int calls = 0;
+  auto radius = [&](const Csr& m, const std::vector<double>& diag) -> double {
+    if (calls++ == 0) return power_radius(A_, diag);  // the finest level is already on the device
+    DevMat tmp = make_mat(m, 1, 1, true, false);
+    return power_radius(tmp, diag);
+  };
+  AmgHost H = amg_setup(std::move(a), std::move(bptr), std::move(bm), nb, radius, opt_.keep_hierarchy);
+  if (opt_.force_amg_breakdown) throw AmgBreakdown("AMG setup: breakdown forced by the caller (test hook)");
+  sizes_ = H.sizes;
+  complexity_ = H.complexity;
+  // coarser level matrices in the V-cycle precision
+  coarse_A_.reserve(H.levels.size());
+  // DOFs per block of every level: nodes on the finest, aggregates below
+  // (6 rigid body modes in 3D, 3 in 2D) when all blocks of a level agree
+  blk_.assign(1, A_.bs);
+  for (size_t l = 1; l < H.levels.size(); ++l) blk_.push_back(uniform_block(H.levels[l].bptr));
+  blk_.push_back(uniform_block(H.coarse_bptr));
+  for (size_t l = 1; l < H.levels.size(); ++l)
+    coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16()));
+  if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
+  else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
+  if (opt_.keep_hierarchy) kept_ = std::make_unique<AmgHost>(std::move(H));
+}
+
+void PcgImpl::setup_amg_device(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val,
+                               int64_t nn, const double* coords, const int64_t* ndof, int bs, bool f32) {
+  std::vector<int64_t> bptr(nn + 1, 0);
+  for (int64_t v = 0; v < nn; ++v) bptr[v + 1] = bptr[v] + ndof[v];
+  int nb = 0;
+  std::vector<double> bm = rigid_body_modes(nn, coords, ndof, nb);
+  if (nb > 6) throw DeviceSetupUnsupported("AMG device setup: more than 6 near-null space vectors");
+  if (bptr[nn] != n) throw DeviceSetupUnsupported("AMG device setup: the nodes' DOF counts do not add up to n");
+  DCsr da = dev_csr_from_lower(n, colptr, rowidx, val, stream_);
+  A_ = make_mat_dev(da, bs, bs, true, f32, false, false, stream_);
+  auto radius = [&](const DCsr& m, int level, const std::vector<double>& diag) -> double {
+    if (level == 0) return power_radius(A_, diag);
+    DevMat tmp;
+    tmp.nbr = m.nrows;
+    tmp.nbc = m.ncols;
+    tmp.nnzb = m.nnz;
+    tmp.group = pick_group(m.nrows ? static_cast<double>(m.nnz) / m.nrows : 1.0);
+    DCsr c = copy_csr(m, stream_);
+    tmp.ptr = std::move(c.ptr);
+    tmp.idx = std::move(c.idx);
+    tmp.v64 = std::move(c.val);
+    return power_radius(tmp, diag);
+  };
+  DevHierarchy H = amg_setup_device(std::move(da), std::move(bptr), bm, nb, radius, opt_.keep_hierarchy, stream_);
+  if (opt_.force_amg_breakdown) throw AmgBreakdown("AMG setup: breakdown forced by the caller (test hook)");
+  sizes_ = H.sizes;
+  complexity_ = H.complexity;
+  coarse_A_.reserve(H.levels.size());
+  blk_.assign(1, bs);
+  for (size_t l = 1; l < H.levels.size(); ++l) blk_.push_back(uniform_block(H.levels[l].bptr));
+  blk_.push_back(uniform_block(H.coarse_bptr));
+  for (size_t l = 1; l < H.levels.size(); ++l)
+    coarse_A_.push_back(
+        make_mat_dev(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16(), stream_));
+  if (f32) build_vcycle_dev<float>(H, v32_, cm32_, cx32_, cb32_);
+  else build_vcycle_dev<double>(H, v64_, cm64_, cx64_, cb64_);
+  if (opt_.keep_hierarchy) kept_ = std::move(H.kept);
+  device_setup_ = true;
 }
 
 PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const double* val, int kind, int64_t nn,
@@ -843,7 +960,6 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
   const auto t0 = std::chrono::steady_clock::now();
   PHIPCHK(hipStreamCreate(&stream_));
   if (kind < 0 || kind > 2) throw std::runtime_error("preconditioner kind must be 0 (AMG), 1 (IC(0)) or 2 (Jacobi)");
-  Csr a = csr_from_lower(n, colptr, rowidx, val);
   // block size of the finest matrix: the common DOF count of the nodes
   int bs = nn > 0 ? static_cast<int>(ndof[0]) : 1;
   int64_t tot = 0;
@@ -853,41 +969,36 @@ PcgImpl::PcgImpl(int64_t n, const int64_t* colptr, const int64_t* rowidx, const 
   }
   if (tot != n || bs < 1 || bs > 3) bs = 1;
   const bool f32 = opt_.vcycle_fp32 != 0;
-  A_ = make_mat(a, bs, bs, true, f32 && kind == 0);
   sizes_.assign(1, n);
-  if (kind == 2) {
-    std::vector<double> d = csr_diag(a);
-    for (double& v : d) v = v != 0.0 ? 1.0 / v : 1.0;
-    jdinv_ = upload(d);
-  } else if (kind == 1) {
-    ic0_ = make_ic0_gpu(a, stream_);
-  } else {
-    std::vector<int64_t> bptr(nn + 1, 0);
-    for (int64_t v = 0; v < nn; ++v) bptr[v + 1] = bptr[v] + ndof[v];
-    int nb = 0;
-    std::vector<double> bm = rigid_body_modes(nn, coords, ndof, nb);
-    int calls = 0;
-    auto radius = [&](const Csr& m, const std::vector<double>& diag) -> double {
-      if (calls++ == 0) return power_radius(A_, diag);  // the finest level is already on the device
-      DevMat tmp = make_mat(m, 1, 1, true, false);
-      return power_radius(tmp, diag);
-    };
-    AmgHost H = amg_setup(std::move(a), std::move(bptr), std::move(bm), nb, radius, opt_.keep_hierarchy);
-    if (opt_.force_amg_breakdown) throw AmgBreakdown("AMG setup: breakdown forced by the caller (test hook)");
-    sizes_ = H.sizes;
-    complexity_ = H.complexity;
-    // coarser level matrices in the V-cycle precision
-    coarse_A_.reserve(H.levels.size());
-    // DOFs per block of every level: nodes on the finest, aggregates below
-    // (6 rigid body modes in 3D, 3 in 2D) when all blocks of a level agree
-    blk_.assign(1, bs);
-    for (size_t l = 1; l < H.levels.size(); ++l) blk_.push_back(uniform_block(H.levels[l].bptr));
-    blk_.push_back(uniform_block(H.coarse_bptr));
-    for (size_t l = 1; l < H.levels.size(); ++l)
-      coarse_A_.push_back(make_mat(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16()));
-    if (f32) build_vcycle<float>(H, v32_, cm32_, cx32_, cb32_);
-    else build_vcycle<double>(H, v64_, cm64_, cx64_, cb64_);
-    if (opt_.keep_hierarchy) kept_ = std::make_unique<AmgHost>(std::move(H));
+  // AMG: the hierarchy is built on the GPU unless RUSTYFEM_AMG_HOST_SETUP=1
+  // asks for the host setup, or the device setup meets a row too long for
+  // its LDS tables (then the host setup runs)
+  const char* hs = std::getenv("RUSTYFEM_AMG_HOST_SETUP");
+  bool done = false;
+  if (kind == 0 && !(hs && *hs && *hs != '0')) {
+    try {
+      setup_amg_device(n, colptr, rowidx, val, nn, coords, ndof, bs, f32);
+      done = true;
+    } catch (const DeviceSetupUnsupported&) {
+      coarse_A_.clear();
+      v32_.clear();
+      v64_.clear();
+      kept_.reset();
+      device_setup_ = false;
+    }
+  }
+  if (!done) {
+    Csr a = csr_from_lower(n, colptr, rowidx, val);
+    A_ = make_mat(a, bs, bs, true, f32 && kind == 0);
+    if (kind == 2) {
+      std::vector<double> d = csr_diag(a);
+      for (double& v : d) v = v != 0.0 ? 1.0 / v : 1.0;
+      jdinv_ = upload(d);
+    } else if (kind == 1) {
+      ic0_ = make_ic0_gpu(a, stream_);
+    } else {
+      setup_amg_host(std::move(a), nn, coords, ndof, f32);
+    }
   }
   x_.alloc(n * 8);
   r_.alloc(n * 8);

@@ -435,8 +435,8 @@ class Pcg:
         return out
 
     def stats(self):
-        o = np.zeros(43)
-        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 43)
+        o = np.zeros(44)
+        lib().rustyfem_pcg_stats(self._h, o.ctypes.data, 44)
         lv = int(o[2])
         return {"setup_ms": o[0], "solve_ms": o[1], "levels": lv, "complexity": o[3], "spmv_ms": o[4],
                 "spmv_bytes": o[5], "coarse": int(o[6]), "iterations": int(o[7]),
@@ -444,7 +444,8 @@ class Pcg:
                 "ic0_forward_levels": int(o[21]), "ic0_backward_levels": int(o[22]), "ic0_attempts": int(o[23]),
                 "fallback": bool(o[24]), "true_rel_residual": o[25], "abandoned_iterations": int(o[26]),
                 "learned_sweeps": int(o[27]), "blocks": [int(b) for b in o[28:28 + min(lv, 12)]],
-                "direct_reason": int(o[40]), "last_direct": bool(o[41]), "direct_rel_residual": o[42]}
+                "direct_reason": int(o[40]), "last_direct": bool(o[41]), "direct_rel_residual": o[42],
+                "setup_on_device": bool(o[43])}
 
     def precondition(self, r):
         """z = M r, one application of the preconditioner."""

@@ -70,3 +70,50 @@ def test_heterogeneous_block_against_oracle_direct(rf, ora, arm):
     xo2, rep = ora.solve_lower(p.n, cp, ri, v, b, "amg", coords, ndof, tol=1e-8)
     assert rep["converged"]
     assert s.stats()["sizes"] == rep["amg_sizes"][:len(s.stats()["sizes"])]
+
+
+def _problem(case):
+    if case == "hex10":
+        return meshes.hex_block(10, nrhs=1)
+    if case == "hetero12":
+        return meshes.hetero_block(12)
+    if case == "tet10":
+        return meshes.tet_block(10, seed=5)
+    return meshes.plate_cpe4(60, 12, 7e10, 0.3, nrhs=1)
+
+
+@pytest.mark.parametrize("case", ["hex10", "hetero12", "tet10", "plate60x12"])
+def test_device_setup_equals_host_setup(rf, case, monkeypatch):
+    """The hierarchy built on the GPU (amg_device.hpp) is the host setup's bit
+    for bit: aggregates, block boundaries, near-null spaces, P~, P and every
+    level matrix, and the solves on the two handles return the same bits."""
+    p = _problem(case)
+    print(f"case amg device vs host setup {case} n={p.n}", flush=True)
+    nodes, coords, ndof, ap, ai = meshes.node_graph(p)
+    cp, ri, v = ac.lower_of(p)
+    monkeypatch.delenv("RUSTYFEM_AMG_HOST_SETUP", raising=False)
+    d = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg", vcycle="fp32", flags=rf.PCG_KEEP_HIERARCHY)
+    monkeypatch.setenv("RUSTYFEM_AMG_HOST_SETUP", "1")
+    h = rf.Pcg(p.n, cp, ri, v, coords=coords, ndof=ndof, krylov="fcg", vcycle="fp32", flags=rf.PCG_KEEP_HIERARCHY)
+    sd, sh = d.stats(), h.stats()
+    assert sd["setup_on_device"] and not sh["setup_on_device"]
+    assert sd["sizes"] == sh["sizes"] and sd["blocks"] == sh["blocks"]
+    for l in range(sd["levels"]):
+        gd, gh = d.level(l), h.level(l)
+        for key in ("nblk", "nagg", "nbm"):
+            assert gd[key] == gh[key]
+        assert np.array_equal(gd["bptr"], gh["bptr"])
+        assert np.array_equal(gd["B"], gh["B"])
+        for key in ("A", "Pt", "P"):
+            if key not in gh:
+                continue
+            md, mh = gd[key], gh[key]
+            assert np.array_equal(md.indptr, mh.indptr) and np.array_equal(md.indices, mh.indices), (l, key)
+            assert np.array_equal(md.data, mh.data), (l, key, np.max(np.abs(md.data - mh.data)))
+        if "agg" in gh:
+            assert np.array_equal(gd["agg"], gh["agg"])
+    b = p.rhs[:, 0].copy()
+    xd, itd, reld, convd = d.solve(b, tol=1e-8)
+    xh, ith, relh, convh = h.solve(b, tol=1e-8)
+    assert convd and convh and itd == ith
+    assert np.array_equal(xd, xh)
