@@ -656,7 +656,16 @@ __global__ __launch_bounds__(WAVE) void k_spgemm_num(int64_t nrows, const int64_
     acc[lo] += a * b;
   };
   const int64_t pe = ap[i + 1];
-  for (int64_t p0 = ap[i]; p0 < pe;) {
+  if (sb == 0) {
+    // long rows of B: every entry p straight from global memory
+    for (int64_t p = ap[i]; p < pe; ++p) {
+      const int32_t k = ai[p];
+      const double a = av[p];
+      for (int64_t q = bp[k] + lane; q < bp[k + 1]; q += WAVE) update(bi[q], a, bv[q]);
+      __syncthreads();
+    }
+  }
+  for (int64_t p0 = sb == 0 ? pe : ap[i]; p0 < pe;) {
     // the B rows of entries p0, p0 + 1, ...: lengths, an inclusive scan over the wave
     const int64_t p = p0 + lane;
     int32_t k = 0;
@@ -739,20 +748,29 @@ DCsr dev_spgemm(const DCsr& a, const DCsr& b, hipStream_t s) {
   const int64_t bound = std::min<int64_t>(static_cast<int64_t>(cand), b.ncols);
   // (at least 256 slots: a full wave can insert 63 keys past the half-full
   // mark, and a probe needs a free slot)
-  const int hs = std::min(LDS_HASH_MAX, std::max(256, next_pow2(2 * std::max<int64_t>(bound, 1))));
-  const size_t sym_lds = static_cast<size_t>(hs) * 4 + static_cast<size_t>(hs / 2) * 4;
+  const int hs_full = std::min(LDS_HASH_MAX, std::max(256, next_pow2(2 * std::max<int64_t>(bound, 1))));
+  // a small table first (the candidate bound counts repeated columns many
+  // times over); the full one only when some row overflows it
+  int hs = std::min(hs_full, 2048);
   DBuf cnt;
   cnt.alloc(std::max<int64_t>(a.nrows, 1) * 8);
-  if (a.nrows)
-    hipLaunchKernelGGL(k_spgemm_sym, dim3(static_cast<unsigned>(a.nrows)), dim3(WAVE), sym_lds, s, a.nrows,
-                       a.ptr.as<int64_t>(), a.idx.as<int32_t>(), b.ptr.as<int64_t>(), b.idx.as<int32_t>(), hs,
-                       cnt.as<int64_t>(), static_cast<const int64_t*>(nullptr), static_cast<int32_t*>(nullptr),
-                       ovf.as<int32_t>());
-  PHIPCHK(hipGetLastError());
-  int32_t over = 0;
-  PHIPCHK(hipMemcpyAsync(&over, ovf.p, 4, hipMemcpyDeviceToHost, s));
-  PHIPCHK(hipStreamSynchronize(s));
-  if (over) throw DeviceSetupUnsupported("AMG device setup: a product row too long for the LDS table");
+  for (;;) {
+    const size_t sym_lds = static_cast<size_t>(hs) * 4 + static_cast<size_t>(hs / 2) * 4;
+    if (a.nrows)
+      hipLaunchKernelGGL(k_spgemm_sym, dim3(static_cast<unsigned>(a.nrows)), dim3(WAVE), sym_lds, s, a.nrows,
+                         a.ptr.as<int64_t>(), a.idx.as<int32_t>(), b.ptr.as<int64_t>(), b.idx.as<int32_t>(), hs,
+                         cnt.as<int64_t>(), static_cast<const int64_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                         ovf.as<int32_t>());
+    PHIPCHK(hipGetLastError());
+    int32_t over = 0;
+    PHIPCHK(hipMemcpyAsync(&over, ovf.p, 4, hipMemcpyDeviceToHost, s));
+    PHIPCHK(hipStreamSynchronize(s));
+    if (!over) break;
+    if (hs == hs_full) throw DeviceSetupUnsupported("AMG device setup: a product row too long for the LDS table");
+    hs = hs_full;
+    PHIPCHK(hipMemsetAsync(ovf.p, 0, 4, s));
+  }
+  const size_t sym_lds = static_cast<size_t>(hs) * 4 + static_cast<size_t>(hs / 2) * 4;
   c.nnz = counts_to_ptr(cnt, c.ptr, a.nrows, s);
   c.idx.alloc(std::max<int64_t>(c.nnz, 1) * 4);
   c.val.alloc(std::max<int64_t>(c.nnz, 1) * 8);
@@ -768,8 +786,16 @@ DCsr dev_spgemm(const DCsr& a, const DCsr& b, hipStream_t s) {
     PHIPCHK(hipGetLastError());
     PHIPCHK(hipStreamSynchronize(s));
     const int ccap = std::max<int>(1, static_cast<int>(longest));
-    // the stage takes what LDS is left (56 KB in all), at least 256 entries
-    const int sb = std::max(256, std::min(4096, (56 * 1024 - ccap * 12 - WAVE * 12 - 8) / 12));
+    // the stage: room for the candidates of a row, at most 1,024 entries
+    // (12 KB) so that many rows stay resident per compute unit, and within
+    // 56 KB of LDS in all
+    // (rows of B of 256 entries and more on average keep the lanes busy
+    // without it: sb = 0 reads them straight from global memory)
+    const bool direct = b.nrows > 0 && b.nnz >= 256 * b.nrows;
+    const int sb = direct ? 0
+                          : std::max(64, std::min({1024,
+                                                   next_pow2(static_cast<int64_t>(std::max<unsigned long long>(cand, 1))),
+                                                   (56 * 1024 - ccap * 12 - WAVE * 12 - 8) / 12}));
     const size_t lds = static_cast<size_t>(ccap) * 12 + static_cast<size_t>(sb) * 12 + WAVE * 8 + (WAVE + 1) * 4;
     hipLaunchKernelGGL(k_spgemm_num, dim3(static_cast<unsigned>(a.nrows)), dim3(WAVE), lds, s, a.nrows,
                        a.ptr.as<int64_t>(), a.idx.as<int32_t>(), a.val.as<double>(), b.ptr.as<int64_t>(),

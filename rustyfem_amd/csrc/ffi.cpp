@@ -15,6 +15,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rustyfem.h"
@@ -48,6 +49,23 @@ Options resolve(const rustyfem_options* o) {
   r.max_refine = o->max_refine >= 0 ? o->max_refine : (r.prec == rf::Precision::MIXED ? 30 : 0);
   r.refine_tol = o->refine_tol > 0 ? o->refine_tol : 1e-10;
   return r;
+}
+
+// f(begin, end) over [0, n) in contiguous parts on the host threads
+template <typename F>
+void parallel_parts(int64_t n, F&& f) {
+  const int64_t t = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), n / 65536 + 1));
+  if (t == 1) {
+    f(int64_t{0}, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + t - 1) / t;
+  for (int64_t k = 0; k < t; ++k) {
+    const int64_t b = std::min(n, k * chunk), e = std::min(n, b + chunk);
+    th.emplace_back([&f, b, e] { f(b, e); });
+  }
+  for (auto& x : th) x.join();
 }
 
 // a number per handle that the process never gives twice
@@ -718,6 +736,30 @@ void rustyfem_stats(const Handle* h, double* out, size_t len) {
 
 const char* rustyfem_last_error(void) { return rf::last_error().c_str(); }
 
+}  // extern "C"
+
+// an allocator whose resize leaves new elements uninitialized (the copies
+// below fill them in parallel)
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <typename U, typename... Args>
+  void construct(U* p, Args&&... args) {
+    if constexpr (sizeof...(Args) == 0)
+      ::new (static_cast<void*>(p)) U;
+    else
+      ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+
+extern "C" {
+
 struct PcgHandle {
   std::unique_ptr<rf::PcgGpu> solver;  // null when the handle solves directly
   int64_t n = 0;
@@ -730,8 +772,10 @@ struct PcgHandle {
   bool last_direct = false;
   double last_direct_rel = -1.0;
   // the inputs, for a direct factor built when a solve does not converge
-  std::vector<int64_t> colptr, rowidx, ndof;
-  std::vector<double> val, coords;
+  std::vector<int64_t> colptr, ndof;
+  std::vector<int64_t, NoInitAlloc<int64_t>> rowidx;
+  std::vector<double, NoInitAlloc<double>> val;
+  std::vector<double> coords;
   int64_t nn = 0;
 };
 
@@ -828,11 +872,23 @@ PcgHandle* rustyfem_pcg_setup_ex(size_t n, const int64_t* colptr, const int64_t*
   *err_eq = -1;
   try {
     // the lower triangle in CSC, natural numbering (src/solver/iterative.rs:74-97)
-    for (size_t c = 0; c < n; ++c) {
-      if (colptr[c + 1] < colptr[c]) throw std::runtime_error("colptr is not monotone");
-      for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p)
-        if (rowidx[p] < static_cast<int64_t>(c) || rowidx[p] >= static_cast<int64_t>(n))
-          throw std::runtime_error("rowidx outside the lower triangle");
+    {
+      std::atomic<int> bad{0};
+      parallel_parts(static_cast<int64_t>(n), [&](int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1 && !bad.load(std::memory_order_relaxed); ++c) {
+          if (colptr[c + 1] < colptr[c]) {
+            bad = 1;
+            break;
+          }
+          for (int64_t p = colptr[c]; p < colptr[c + 1]; ++p)
+            if (rowidx[p] < c || rowidx[p] >= static_cast<int64_t>(n)) {
+              bad = 2;
+              break;
+            }
+        }
+      });
+      if (bad == 1) throw std::runtime_error("colptr is not monotone");
+      if (bad == 2) throw std::runtime_error("rowidx outside the lower triangle");
     }
     int64_t tot = 0;
     for (size_t v = 0; v < nn; ++v) {
@@ -855,9 +911,16 @@ PcgHandle* rustyfem_pcg_setup_ex(size_t n, const int64_t* colptr, const int64_t*
     h->n = static_cast<int64_t>(n);
     h->fallback = (flags & RUSTYFEM_PCG_NO_DIRECT_FALLBACK) == 0;
     if (h->fallback) {
+      // a copy of the matrix for the direct fallbacks, in parallel parts
       h->colptr.assign(colptr, colptr + n + 1);
-      h->rowidx.assign(rowidx, rowidx + colptr[n]);
-      h->val.assign(val, val + colptr[n]);
+      h->rowidx.resize(colptr[n]);
+      h->val.resize(colptr[n]);
+      parallel_parts(colptr[n], [&](int64_t b, int64_t e) {
+        if (e > b) {
+          std::memcpy(h->rowidx.data() + b, rowidx + b, (e - b) * sizeof(int64_t));
+          std::memcpy(h->val.data() + b, val + b, (e - b) * sizeof(double));
+        }
+      });
       h->nn = static_cast<int64_t>(nn);
       h->ndof.assign(ndof, ndof + nn);
       h->coords.assign(coords, coords + 3 * nn);

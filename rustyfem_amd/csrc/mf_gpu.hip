@@ -820,12 +820,13 @@ __global__ __launch_bounds__(64 * NW) void k_sfwd_rows(const FrontDesc* __restri
   const double* yf = ysrc + d.toff;
   // the lane's row (clamped into the front) of the wave's columns
   const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * d.m + min(r, d.m - 1);
+  // all of the wave's loads of L first (one memory latency, once for every
+  // chunk of right-hand sides the workgroup takes), then y
+  T lv[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) lv[c] = F[static_cast<int64_t>(min(cb + c, W - 1)) * d.m];
   for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
-    // all of the wave's loads of L first (one memory latency), then y
-    T lv[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) lv[c] = F[static_cast<int64_t>(min(cb + c, W - 1)) * d.m];
     __syncthreads();
     for (int idx = tid; idx < W * R; idx += 64 * NW) {
       const int i = idx % W, j = idx / W;
@@ -894,6 +895,15 @@ __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__
   int rowc[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) rowc[s] = min(tl.r0 + rl + 16 * s, rend - 1);
+  // the tile of L once, for every chunk of right-hand sides the workgroup takes
+  T lv[16][4];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int cc = min(tl.c0 + 64 * wave + 4 * q + cg, tl.c1 - 1);
+    const T* col = F + static_cast<int64_t>(cc) * d.m;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) lv[q][s] = col[rowc[s]];
+  }
   for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
     double tr[4][R];
@@ -904,14 +914,6 @@ __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__
         const int row = tl.r0 + rl + 16 * s;
         tr[s][j] = (row < rend && j < nr) ? tf[(j0 + j) * tv + row] : 0.0;
       }
-    T lv[16][4];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int cc = min(tl.c0 + 64 * wave + 4 * q + cg, tl.c1 - 1);
-      const T* col = F + static_cast<int64_t>(cc) * d.m;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) lv[q][s] = col[rowc[s]];
-    }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       double acc[R];
@@ -1134,11 +1136,11 @@ __global__ __launch_bounds__(64 * NW) void k_sfwd_inv(const FrontDesc* __restric
   const double* tf = t + d.toff + tk.J0;
   double* yf = ys + d.toff + tk.J0;
   const double* xr = X + min(i, W - 1);
+  double xv[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, jend - 1)) * W];
   for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
-    double xv[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, jend - 1)) * W];
     __syncthreads();
     for (int idx = tid; idx < jend * R; idx += 64 * NW) {
       const int q = idx % jend, j = idx / jend;
@@ -1228,11 +1230,11 @@ __global__ __launch_bounds__(64 * NW) void k_sbwd_inv(const FrontDesc* __restric
   const double* zf = ys + d.toff + tk.J0;
   double* tf = t + d.toff + tk.J0;
   const double* xr = Xt + min(i, W - 1);
+  double xv[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, W - 1)) * W];
   for (int j0 = static_cast<int>(blockIdx.z) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.z)) {
     const int nr = min(R, nrhs - j0);
-    double xv[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) xv[c] = xr[static_cast<int64_t>(min(cb + c, W - 1)) * W];
     __syncthreads();
     for (int idx = tid; idx < span * R; idx += 64 * NW) {
       const int q = a0 + idx % span, j = idx / span;
@@ -2197,19 +2199,28 @@ class Impl final : public GpuFactor {
     }
   }
 
+  // Chunks of right-hand sides per launch: a launch with enough workgroups
+  // for the GPU takes every chunk in each workgroup (the factor is read once),
+  // a small one spreads the chunks over workgroups (more parallelism at the
+  // top of the tree, the factor read once per chunk).
+  static constexpr int64_t kChunkAll = 512;
+  static unsigned rhs_chunks(int64_t blocks, unsigned nch) { return blocks >= kChunkAll ? 1u : nch; }
+
   // the super-block steps of one level, NW waves per workgroup
   template <int R, int NW>
   void sb_fwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
     const int nbk = SBW / NB;
     for (const SbFwdStep& st : lp.sfwd) {
       if (st.chain.count > 0) {
-        hipLaunchKernelGGL((k_sfwd_inv<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
-                           s_, d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, w_.t, w_.ys, tv, nrhs);
+        hipLaunchKernelGGL((k_sfwd_inv<R, NW>),
+                           dim3(static_cast<unsigned>(st.chain.count), nbk, rhs_chunks(st.chain.count * nbk, nch)),
+                           dim3(64 * NW), 0, s_, d_fd_, d_sbt_ + st.chain.begin, d_xinv_, d_sboff_, w_.t, w_.ys, tv,
+                           nrhs);
         HIPCHK(hipGetLastError());
       }
       if (st.rows.count > 0) {
-        hipLaunchKernelGGL((k_sfwd_rows<T, R, NW>), dim3(grid(st.rows.count), nch), dim3(64 * NW), 0, s_, d_fd_,
-                           d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
+        hipLaunchKernelGGL((k_sfwd_rows<T, R, NW>), dim3(grid(st.rows.count), rhs_chunks(st.rows.count, nch)),
+                           dim3(64 * NW), 0, s_, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
         HIPCHK(hipGetLastError());
       }
     }
@@ -2220,16 +2231,18 @@ class Impl final : public GpuFactor {
     const int nbk = SBW / NB;
     for (const SbBwdStep& st : lp.sbwd) {
       if (st.part.count > 0) {
-        hipLaunchKernelGGL((k_sbwd_part<T, R>), dim3(grid(st.part.count), nch), dim3(256), 0, s_, d_fd_,
-                           d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
+        hipLaunchKernelGGL((k_sbwd_part<T, R>), dim3(grid(st.part.count), rhs_chunks(st.part.count, nch)), dim3(256), 0,
+                           s_, d_fd_, d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
         HIPCHK(hipGetLastError());
       }
       if (st.chain.count > 0) {
         hipLaunchKernelGGL((k_sbwd_comb<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
                            s_, d_fd_, d_sbt_ + st.chain.begin, w_.part, w_.t, w_.ys, tv, nrhs);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL((k_sbwd_inv<R, NW>), dim3(static_cast<unsigned>(st.chain.count), nbk, nch), dim3(64 * NW), 0,
-                           s_, d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, w_.ys, w_.t, tv, nrhs);
+        hipLaunchKernelGGL((k_sbwd_inv<R, NW>),
+                           dim3(static_cast<unsigned>(st.chain.count), nbk, rhs_chunks(st.chain.count * nbk, nch)),
+                           dim3(64 * NW), 0, s_, d_fd_, d_sbt_ + st.chain.begin, d_xinvt_, d_sboff_, w_.ys, w_.t, tv,
+                           nrhs);
         HIPCHK(hipGetLastError());
       }
     }

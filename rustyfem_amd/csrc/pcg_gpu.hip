@@ -922,8 +922,20 @@ void PcgImpl::setup_amg_device(int64_t n, const int64_t* colptr, const int64_t* 
   std::vector<double> bm = rigid_body_modes(nn, coords, ndof, nb);
   if (nb > 6) throw DeviceSetupUnsupported("AMG device setup: more than 6 near-null space vectors");
   if (bptr[nn] != n) throw DeviceSetupUnsupported("AMG device setup: the nodes' DOF counts do not add up to n");
+  const char* tenv = std::getenv("RUSTYFEM_AMG_TIMING");
+  const bool timing = tenv && *tenv && *tenv != '0';
+  auto t0 = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (!timing) return;
+    PHIPCHK(hipStreamSynchronize(stream_));
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[amg-device] %s %.3f s\n", what, std::chrono::duration<double>(t1 - t0).count());
+    t0 = t1;
+  };
   DCsr da = dev_csr_from_lower(n, colptr, rowidx, val, stream_);
+  lap("full CSR from the lower triangle");
   A_ = make_mat_dev(da, bs, bs, true, f32, false, false, stream_);
+  lap("finest BSR");
   auto radius = [&](const DCsr& m, int level, const std::vector<double>& diag) -> double {
     if (level == 0) return power_radius(A_, diag);
     DevMat tmp;
@@ -937,7 +949,9 @@ void PcgImpl::setup_amg_device(int64_t n, const int64_t* colptr, const int64_t* 
     tmp.v64 = std::move(c.val);
     return power_radius(tmp, diag);
   };
+  lap("rigid body modes");
   DevHierarchy H = amg_setup_device(std::move(da), std::move(bptr), bm, nb, radius, opt_.keep_hierarchy, stream_);
+  lap("hierarchy");
   if (opt_.force_amg_breakdown) throw AmgBreakdown("AMG setup: breakdown forced by the caller (test hook)");
   sizes_ = H.sizes;
   complexity_ = H.complexity;
@@ -950,6 +964,7 @@ void PcgImpl::setup_amg_device(int64_t n, const int64_t* colptr, const int64_t* 
         make_mat_dev(H.levels[l].A, blk_[l], blk_[l], !f32, f32, true, f32 && vcycle_bf16(), stream_));
   if (f32) build_vcycle_dev<float>(H, v32_, cm32_, cx32_, cb32_);
   else build_vcycle_dev<double>(H, v64_, cm64_, cx64_, cb64_);
+  lap("V-cycle operators");
   if (opt_.keep_hierarchy) kept_ = std::move(H.kept);
   device_setup_ = true;
 }
