@@ -1269,6 +1269,10 @@ __global__ __launch_bounds__(256) void k_fwd_finish_ys(const FrontDesc* __restri
 
 // ------------------------------------------------------------- refinement
 
+// the failing-pivot word to "none" (a kernel, not a memset node, so that the
+// factorization can be captured into a graph)
+__global__ void k_err_reset(unsigned long long* err) { *err = ULLONG_MAX; }
+
 __global__ void k_gather(const int64_t* __restrict__ map, const double* __restrict__ src, double* __restrict__ dst,
                          int64_t cnt) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1464,6 +1468,8 @@ class Impl final : public GpuFactor {
     }
     const char* ng = std::getenv("RUSTYFEM_NO_GRAPH");
     use_graphs_ = !(ng && *ng && *ng != '0');
+    const char* nf = std::getenv("RUSTYFEM_NO_FACTOR_GRAPH");
+    factor_graph_ = use_graphs_ && !(nf && *nf && *nf != '0');
     HIPCHK(hipEventCreate(&e0_));
     HIPCHK(hipEventCreate(&e1_));
     n_ = P_.n;
@@ -1584,6 +1590,7 @@ class Impl final : public GpuFactor {
     (void)hipStreamSynchronize(s2_);
     (void)hipStreamSynchronize(s_);
     drop_graphs();
+    if (fgraph_) (void)hipGraphExecDestroy(fgraph_);
     void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
                     d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_c_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
@@ -1598,14 +1605,67 @@ class Impl final : public GpuFactor {
     (void)hipStreamDestroy(s_);
   }
 
+  // The factorization is a fixed sequence of launches on fixed buffers once
+  // the GEMM kernels are tuned: from the second call on it is captured once
+  // into a HIP graph (both streams, their events as graph edges) and
+  // replayed.  Kernel timing (per-GEMM events) and RUSTYFEM_NO_GRAPH /
+  // RUSTYFEM_NO_FACTOR_GRAPH keep it eager.
   int64_t factor(const double* vals, bool on_device) override {
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (vals) HIPCHK(hipMemcpyAsync(d_lval_, vals, nlower_ * sizeof(double), kind, s_));
-    HIPCHK(hipMemsetAsync(d_err_, 0xFF, sizeof(unsigned long long), s_));
-    HIPCHK(hipEventRecord(e0_, s_));
+    const bool graph = factor_graph_ && !ktime_ && (tuned_ || !autotune_);
+    if (graph) {
+      if (!fgraph_) {
+        HIPCHK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+          factor_launches(false);
+        } catch (...) {
+          hipGraph_t dummy = nullptr;
+          (void)hipStreamEndCapture(s_, &dummy);
+          if (dummy) (void)hipGraphDestroy(dummy);
+          rest_pending_ = false;
+          throw;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(s_, &g));
+        HIPCHK(hipGraphInstantiate(&fgraph_, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        fgraph_flops_ = tm_.gemm_flops;
+      }
+      HIPCHK(hipEventRecord(e0_, s_));
+      HIPCHK(hipGraphLaunch(fgraph_, s_));
+      HIPCHK(hipEventRecord(e1_, s_));
+      tm_.gemm_flops = fgraph_flops_;
+    } else {
+      HIPCHK(hipEventRecord(e0_, s_));
+      factor_launches(true);
+      HIPCHK(hipEventRecord(e1_, s_));
+    }
+    unsigned long long err = 0;
+    HIPCHK(hipMemcpyAsync(&err, d_err_, sizeof err, hipMemcpyDeviceToHost, s_));
+    HIPCHK(hipStreamSynchronize(s_));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
+    tm_.factor_ms = ms;
+    if (ktime_) {
+      tm_.gemm_ms = 0;
+      for (int64_t i = 0; i < nev_; ++i) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, evs_[2 * i], evs_[2 * i + 1]));
+        tm_.gemm_ms += t;
+      }
+      tm_.gemm_launches = static_cast<double>(nev_);
+    }
+    return err == ULLONG_MAX ? -1 : static_cast<int64_t>(err);
+  }
+
+  // the launches of one numeric factorization on s_ (and s2_, joined back
+  // to s_ at the end); may_tune: run the GEMM autotuner when it is due
+  void factor_launches(bool may_tune) {
+    launch(k_err_reset, 1, 1, d_err_);
     launch(k_gather, (n_ + 255) / 256, 256, d_dpos_, d_lval_, d_adiag_, n_);
     launch(k_gather, (nfull_ + 255) / 256, 256, d_gmap_, d_lval_, d_av_, nfull_);
-    if (autotune_ && !tuned_) tune_gemm();
+    if (may_tune && autotune_ && !tuned_) tune_gemm();
     nev_ = 0;
     tm_.gemm_flops = 0;
     // each level's fronts are cleared before the level: level 0 here, level
@@ -1694,27 +1754,15 @@ class Impl final : public GpuFactor {
       }
     }
     join_rest();
+    if (lookahead_) {
+      // the second stream back into the first (a capture must end joined)
+      HIPCHK(hipEventRecord(ev_rest_, s2_));
+      HIPCHK(hipStreamWaitEvent(s_, ev_rest_, 0));
+    }
     // explicit inverses of the super-blocks for the solves
     if (!legacy_solve_ && !chain_solve_)
       launch(k_sbinv<T>, static_cast<int64_t>(P_.sbinv.size()), 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
              d_sboff_, d_xinv_, d_xinvt_);
-    HIPCHK(hipEventRecord(e1_, s_));
-    unsigned long long err = 0;
-    HIPCHK(hipMemcpyAsync(&err, d_err_, sizeof err, hipMemcpyDeviceToHost, s_));
-    HIPCHK(hipStreamSynchronize(s_));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0_, e1_));
-    tm_.factor_ms = ms;
-    if (ktime_) {
-      tm_.gemm_ms = 0;
-      for (int64_t i = 0; i < nev_; ++i) {
-        float t = 0;
-        HIPCHK(hipEventElapsedTime(&t, evs_[2 * i], evs_[2 * i + 1]));
-        tm_.gemm_ms += t;
-      }
-      tm_.gemm_launches = static_cast<double>(nev_);
-    }
-    return err == ULLONG_MAX ? -1 : static_cast<int64_t>(err);
   }
 
   void set_caller_map(const std::vector<int64_t>& eq, const std::vector<double>& iso_diag) override {
@@ -2338,6 +2386,9 @@ class Impl final : public GpuFactor {
   };
   std::vector<GraphEntry> graphs_;
   bool use_graphs_ = true;
+  bool factor_graph_ = true;           // capture and replay the refactorization
+  hipGraphExec_t fgraph_ = nullptr;
+  double fgraph_flops_ = 0;
   bool ktime_ = false;
   std::vector<hipEvent_t> evs_;
   int64_t nev_ = 0;

@@ -239,3 +239,29 @@ def test_lookahead_gives_identical_factor(rf, monkeypatch, precision):
     # a refactor on the same handle (the lookahead events are reused) as well
     h_l.refactor(p.data)
     assert np.array_equal(h_l.solve(p.rhs), h_s.solve(p.rhs))
+
+
+def test_refactor_graph_replays_match_eager(rf, monkeypatch):
+    """From the second factorization on, the refactorization is one captured
+    HIP graph replayed per call: the factors (seen through the solutions and
+    a failing pivot) equal those of the eager launches bit for bit."""
+    p = meshes.hex_block(10, nrhs=2)
+    print(f"case refactor graph vs eager n={p.n}", flush=True)
+    out = {}
+    for mode in ("graph", "eager"):
+        if mode == "eager":
+            monkeypatch.setenv("RUSTYFEM_NO_FACTOR_GRAPH", "1")
+        else:
+            monkeypatch.delenv("RUSTYFEM_NO_FACTOR_GRAPH", raising=False)
+        h, err, _ = rf.Handle.factor(p, precision=0)
+        assert err == -1
+        xs = []
+        for k in range(3):
+            assert h.refactor(p.data * (1.0 + 0.25 * k)) == -1
+            xs.append(h.solve(p.rhs))
+        xs.append(h.refactor(-p.data))
+        out[mode] = xs
+        h.free()
+    for a, c in zip(out["graph"][:3], out["eager"][:3]):
+        np.testing.assert_array_equal(a, c)
+    assert out["graph"][3] == out["eager"][3] >= 0
