@@ -895,6 +895,12 @@ __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__
   int rowc[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) rowc[s] = min(tl.r0 + rl + 16 * s, rend - 1);
+  // the wave's groups of four columns that hold a column of the tile: a wave
+  // past the tile's columns ends here, and the groups past them are loaded
+  // (clamped, so that all loads still go out together) but not multiplied or
+  // stored (the combination reads only the tile's columns)
+  const int nq = min(16, max(0, (tl.c1 - tl.c0 - 64 * wave + 3) / 4));
+  if (nq == 0) return;
   // the tile of L once, for every chunk of right-hand sides the workgroup takes
   T lv[16][4];
 #pragma unroll
@@ -916,20 +922,24 @@ __global__ __launch_bounds__(256) void k_sbwd_part(const FrontDesc* __restrict__
       }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      double acc[R];
+      // a uniform predicate, not a break: the loop stays unrolled and lv in
+      // registers
+      if (q < nq) {
+        double acc[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        acc[j] = 0.0;
+        for (int j = 0; j < R; ++j) {
+          acc[j] = 0.0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc[j] = fma(static_cast<double>(lv[q][s]), tr[s][j], acc[j]);
-        acc[j] = row16_sum(acc[j]);
-      }
-      const int cl = 64 * wave + 4 * q + cg;
-      if (rl == 0) {
-        const bool in = tl.c0 + cl < tl.c1;
+          for (int s = 0; s < 4; ++s) acc[j] = fma(static_cast<double>(lv[q][s]), tr[s][j], acc[j]);
+          acc[j] = row16_sum(acc[j]);
+        }
+        const int cl = 64 * wave + 4 * q + cg;
+        if (rl == 0) {
+          const bool in = tl.c0 + cl < tl.c1;
 #pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (j < nr) out[(j0 + j) * BC + cl] = in ? acc[j] : 0.0;
+          for (int j = 0; j < R; ++j)
+            if (j < nr) out[(j0 + j) * BC + cl] = in ? acc[j] : 0.0;
+        }
       }
     }
   }
