@@ -36,12 +36,15 @@ __device__ __forceinline__ void block_ldlt_blk(double (*A)[NB + 1], double (*V)[
   constexpr int BS = 16, NBK = NB / BS;
   const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // identity beyond jb, so that every step below is uniform
+  // identity beyond jb, so that every step below is uniform; the 16-column
+  // blocks past jb (nbe on) stay identity and are skipped
+  const int nbe = __builtin_amdgcn_readfirstlane((jb + BS - 1) / BS);
   if (tid < NB && tid >= jb) A[tid][tid] = 1.0;
   for (int idx = tid; idx < NB * NB; idx += 256) V[idx / NB][idx % NB] = 0.0;
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < NBK; ++b) {
+    if (b >= nbe) continue;
     const int c0 = BS * b;
     if (wave == 0) {
       double r[BS];
@@ -71,8 +74,8 @@ __device__ __forceinline__ void block_ldlt_blk(double (*A)[NB + 1], double (*V)[
       }
     }
     __syncthreads();
-    // trailing lower tiles (ti, tc), b < tc <= ti < NBK, one per wave
-    const int nt = NBK - 1 - b;
+    // trailing lower tiles (ti, tc), b < tc <= ti < nbe, one per wave
+    const int nt = nbe - 1 - b;
     for (int tt = wave; tt < nt * (nt + 1) / 2; tt += 4) {
       int ti = b + 1, rem = tt;
       while (rem >= ti - b) {
@@ -114,9 +117,9 @@ __device__ __forceinline__ void block_ldlt_blk(double (*A)[NB + 1], double (*V)[
     for (int i = 0; i < BS; ++i) V[r0 + l16][r0 + i] = x[i];
   }
   __syncthreads();
-  for (int dist = 1; dist < NBK; ++dist) {
+  for (int dist = 1; dist < nbe; ++dist) {
     const int a = dist + wave;
-    if (a < NBK) {
+    if (a < nbe) {
       const int b = a - dist;
       // T = sum_{c=b}^{a-1} L_ac V_cb; register r holds T[g + 4 r][l16]
       M::acc_t t = M::acc_t{0, 0, 0, 0};
@@ -170,7 +173,8 @@ struct PanelPass {
 // A_BB = F_BB - C_BB, then block_ldlt_blk; writes D and V = L_BB^-1 (the
 // arena's pivot block stays as assembled).
 template <typename T>
-__global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pdiag(
+    const FrontDesc* __restrict__ fd, const PanelTile* __restrict__ tiles,
                                                const T* __restrict__ arena, double* __restrict__ dpiv,
                                                const double* __restrict__ adiag, double* __restrict__ vbuf,
                                                unsigned long long* __restrict__ err) {
@@ -180,10 +184,13 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
   constexpr int YBYTES = PK * (NB + 2) * static_cast<int>(sizeof(T)) + PK * 8;
   constexpr int ABYTES = NB * (NB + 1) * 8;  // Ad, factored in place
   constexpr int TOFF = (ABYTES + 15) / 16 * 16;  // the inverse's scratch after Ad
-  constexpr int LBYTES = TOFF + 4 * 256 * 8;
+  constexpr int VOFF = TOFF + 4 * 256 * 8;         // then Vs
+  constexpr int LBYTES = VOFF + NB * (NB + 2) * 8;
+  // the update's staging and the factorization's buffers share one region
+  // (75 KB in all), so that two workgroups fit in a CU's LDS
   __shared__ __attribute__((aligned(16))) unsigned char smem[YBYTES > LBYTES ? YBYTES : LBYTES];
-  __shared__ __attribute__((aligned(16))) double Vs[NB][NB + 2];  // Vs[q][i] = L^-1[i][q]
   __shared__ double cbuf[NB], dd[NB];
+  double(*Vs)[NB + 2] = reinterpret_cast<double(*)[NB + 2]>(smem + VOFF);  // Vs[q][i] = L^-1[i][q]
   T(*Yk)[NB + 2] = reinterpret_cast<T(*)[NB + 2]>(smem);                            // Yk[k][i] = L[j0 + i, p0 + k]
   double* Ds = reinterpret_cast<double*>(smem + PK * (NB + 2) * sizeof(T));          // D[p0 + k]
   double(*Ad)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);                  // after the update
@@ -197,15 +204,21 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
   // the original diagonal of lane i's pivot (for the test after the
   // elimination) and the block's own entries (lower triangle, mirrored)
   const double orig = lane < jb ? adiag[static_cast<int64_t>(d.start) + t.j0 + lane] : 1.0;
+  // (read before the update in FP32; in FP64 after it, where the 32 registers
+  // they take would push the update past two waves per SIMD)
+  constexpr bool kEarly = sizeof(T) == 4;
   T fb[4][4];
+  auto load_fb = [&]() {
 #pragma unroll
-  for (int bj = 0; bj < 4; ++bj)
+    for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = bj * 16 + M::orow(lane, r);
-      const int hi = min(max(i, q), jb - 1), lo = min(min(i, q), jb - 1);
-      fb[bj][r] = F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m];
-    }
+      for (int r = 0; r < 4; ++r) {
+        const int q = bj * 16 + M::orow(lane, r);
+        const int hi = min(max(i, q), jb - 1), lo = min(min(i, q), jb - 1);
+        fb[bj][r] = F[t.j0 + hi + static_cast<int64_t>(t.j0 + lo) * m];
+      }
+  };
+  if constexpr (kEarly) load_fb();
   M::acc_t cb[4];
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj) cb[bj] = M::acc_t{0, 0, 0, 0};
@@ -255,6 +268,7 @@ __global__ __launch_bounds__(256) void k_pdiag(const FrontDesc* __restrict__ fd,
       }
     }
   }
+  if constexpr (!kEarly) load_fb();
   __syncthreads();
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj)
