@@ -410,7 +410,7 @@ bool batch_numeric(BatchHandle* h, const double* const* data, int64_t* err) {
 
 // entries of rustyfem_stats / rustyfem_batch_stats that come from the
 // factorization's timings (the layout is documented in rustyfem.h)
-constexpr size_t NSTATS = 25;
+constexpr size_t NSTATS = 26;
 void fill_timings(const rf::Timings& t, double* v) {
   v[0] = t.factor_ms;
   v[1] = t.solve_ms;
@@ -429,6 +429,7 @@ void fill_timings(const rf::Timings& t, double* v) {
   v[22] = t.fell_back;
   v[23] = t.fallback_rel;
   v[24] = t.q_launches;
+  v[25] = t.small_levels;
 }
 
 }  // namespace

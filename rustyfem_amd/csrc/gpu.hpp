@@ -30,6 +30,8 @@ struct Timings {
   // fallback (an FP64 factorization); the relative residual it reached
   int32_t fell_back = 0;
   double fallback_rel = -1;
+  // tree levels whose solves run through the fused small-front kernels
+  double small_levels = 0;
 };
 
 // Solves in internal numbering on device vectors (b, x: n x nrhs), returns the

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -1268,6 +1269,232 @@ __global__ __launch_bounds__(64 * NW) void k_sbwd_inv(const FrontDesc* __restric
   }
 }
 
+// ------------------------------------------------ fused solves of small fronts
+//
+// Levels whose fronts all have at most SMM rows and SMK pivots (the lower
+// levels of 2D trees, every level of a batch of small plates) are solved by
+// one wave per front and one launch per level and direction, instead of five
+// or six launches of four-wave workgroups.  Each step repeats the arithmetic
+// of the four-wave kernel it replaces, in the same order (forward:
+// k_fwd_init, k_fwd_child per slot, k_sfwd_inv, k_fwd_finish_ys, k_sfwd_rows;
+// backward: k_bwd_init, k_sbwd_part, k_sbwd_comb, k_sbwd_inv, k_bwd_finish):
+// a lane forms the four waves' partial sums of its row one after another and
+// adds them in the waves' pairwise order, so the results are bit-identical.
+// The front's vector stays in LDS between the steps; the LDS is sized per
+// level (ldm rows, ldk pivots).  Such a front has one super-block
+// (SMK <= SBW), at most SMM / PRT backward row tiles and one 256-column chunk.
+
+// sum over the four waves' column ranges of a 64-row block of the four-wave
+// kernels: v[w][j] = sum_{c in [cb_w, ce_w)} a(c) s[c][j], in order, then
+// (v0 + v2) + (v1 + v3).  lo is the block's first column, span its width.
+template <int R, typename LoadA>
+__device__ __forceinline__ void four_wave_sums(LoadA a, const double* __restrict__ s, int ld, int lo, int span,
+                                               double (&out)[R]) {
+  const int cw = (span + 3) / 4;
+  double v[4][R];
+#pragma unroll
+  for (int ww = 0; ww < 4; ++ww) {
+    const int cb = lo + ww * cw, ce = min(lo + span, cb + cw);
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[ww][j] = 0.0;
+#pragma unroll 4
+    for (int c = cb; c < ce; ++c) {
+      const double av = a(c);
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[ww][j] = fma(av, s[c * ld + j], v[ww][j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) out[j] = (v[0][j] + v[2][j]) + (v[1][j] + v[3][j]);
+}
+
+// forward, one wave per front: t = x[pivots] (0 below), += the children's
+// update vectors (slot 0, then 1), y = X t[0:k], x[pivots] = y / D,
+// t[k:m] -= L[k:m, 0:k] y, and t[k:m] out for the parent
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_small_fwd(const FrontDesc* __restrict__ fd, const SmallFront* __restrict__ sf,
+                                                  const int32_t* __restrict__ cmap, const T* __restrict__ arena,
+                                                  const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
+                                                  const double* __restrict__ dpiv, double* __restrict__ x,
+                                                  double* __restrict__ t, int64_t n, int64_t tv, int nrhs, int ldm) {
+  static_assert(SMK <= SBW && SMK == sb_maxw<4>() && SRB == 64, "one super-block per front, 64-row update tiles");
+  constexpr int LD = R + 1;
+  extern __shared__ double sh[];
+  double* ts = sh;              // ldm x LD: the front's vector
+  double* ys = sh + ldm * LD;   // k x LD: y
+  const SmallFront sm = sf[blockIdx.x];
+  const FrontDesc d = fd[sm.front];
+  const int lane = threadIdx.x;
+  const int m = d.m, k = d.k, W = d.k;
+  const double* X = k > 0 ? xinv + sboff[sm.sb] : nullptr;
+  for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = lane; idx < m * R; idx += 64) {
+      const int i = idx % m, j = idx / m;
+      ts[i * LD + j] = (i < k && j < nr) ? x[static_cast<int64_t>(j0 + j) * n + d.start + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl) {
+      if (sm.child[sl] < 0) continue;
+      const FrontDesc c = fd[sm.child[sl]];
+      const int32_t* cm = cmap + c.cmap_off;
+      for (int idx = lane; idx < c.u * R; idx += 64) {
+        const int xx = idx % c.u, j = idx / c.u;
+        if (j < nr) ts[cm[xx] * LD + j] += t[static_cast<int64_t>(j0 + j) * tv + c.toff + c.k + xx];
+      }
+      __syncthreads();
+    }
+    if (k > 0) {
+      // y = X t[0:k] by 64-row blocks (k_sfwd_inv)
+      for (int a0 = 0; a0 < W; a0 += 64) {
+        const int i = a0 + lane;
+        if (i < W) {
+          const double* xr = X + i;
+          double y[R];
+          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * W]; }, ts, LD, 0, min(W, a0 + 64), y);
+#pragma unroll
+          for (int j = 0; j < R; ++j) ys[i * LD + j] = j < nr ? y[j] : 0.0;
+        }
+      }
+      __syncthreads();
+      // x[pivots] = y / D (k_fwd_finish_ys)
+      for (int idx = lane; idx < k * R; idx += 64) {
+        const int i = idx % k, j = idx / k;
+        if (j < nr) x[static_cast<int64_t>(j0 + j) * n + d.start + i] = ys[i * LD + j] / dpiv[d.start + i];
+      }
+      // t[k:m] -= L[k:m, 0:k] y by 64-row tiles (k_sfwd_rows)
+      for (int r0 = k; r0 < m; r0 += SRB) {
+        const int r = r0 + lane;
+        if (r < m) {
+          const T* F = arena + d.off + r;
+          double v[R];
+          four_wave_sums<R>([&](int c) { return static_cast<double>(F[static_cast<int64_t>(c) * m]); }, ys, LD, 0, W,
+                            v);
+          for (int j = 0; j < nr; ++j) ts[r * LD + j] -= v[j];
+        }
+      }
+      __syncthreads();
+    }
+    // the update vector for the parent
+    const int u = m - k;
+    for (int idx = lane; idx < u * R; idx += 64) {
+      const int i = k + idx % u, j = idx / u;
+      if (j < nr) t[static_cast<int64_t>(j0 + j) * tv + d.toff + i] = ts[i * LD + j];
+    }
+  }
+}
+
+// backward, one wave per front: t = x[pivots] and the parent's final entries
+// below, z = t[0:k] - L[k:m, 0:k]^T t[k:m] (partial sums per 64-row tile,
+// combined in the order of k_sbwd_comb), t[0:k] = X^T z, x[pivots] = t[0:k],
+// and the whole t out for the children
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_small_bwd(const FrontDesc* __restrict__ fd, const SmallFront* __restrict__ sf,
+                                                  const int32_t* __restrict__ cmap, const T* __restrict__ arena,
+                                                  const double* __restrict__ xinvt, const int64_t* __restrict__ sboff,
+                                                  double* __restrict__ x, double* __restrict__ t, int64_t n, int64_t tv,
+                                                  int nrhs, int ldm, int ldk) {
+  static_assert(SMK <= SBW && SMK == sb_maxw<4>() && SMK <= BC && PRT == 64,
+                "one super-block and one column chunk per front");
+  constexpr int LD = R + 1;
+  extern __shared__ double sh[];
+  double* ts = sh;                   // ldm x LD: the front's vector
+  double* zs = sh + ldm * LD;        // ldk x LD: z
+  double* part = zs + ldk * LD;      // (ldm / PRT rounded up) x ldk x LD: partial sums per row tile
+  const SmallFront sm = sf[blockIdx.x];
+  const FrontDesc d = fd[sm.front];
+  const int lane = threadIdx.x;
+  const int m = d.m, k = d.k, W = d.k;
+  const int64_t ptoff = d.parent >= 0 ? fd[d.parent].toff : 0;
+  const T* F = arena + d.off;
+  const int nrc = (m - k + PRT - 1) / PRT;
+  for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
+    const int nr = min(R, nrhs - j0);
+    __syncthreads();
+    for (int idx = lane; idx < m * R; idx += 64) {
+      const int i = idx % m, j = idx / m;
+      double v = 0.0;
+      if (j < nr)
+        v = i < k ? x[static_cast<int64_t>(j0 + j) * n + d.start + i]
+                  : t[static_cast<int64_t>(j0 + j) * tv + ptoff + cmap[d.cmap_off + i - k]];
+      ts[i * LD + j] = v;
+    }
+    __syncthreads();
+    if (k > 0) {
+      // partial sums of 64-row tiles (k_sbwd_part): 16 lanes per column,
+      // rows rl + 16 s of the tile, summed across the DPP row; four columns
+      // per step
+      const int rl = lane & 15, cg = lane >> 4;
+      for (int rc = 0; rc < nrc; ++rc) {
+        const int r0 = k + rc * PRT, rend = min(r0 + PRT, m);
+        for (int c4 = 0; c4 < k; c4 += 4) {
+          const int cl = c4 + cg;
+          const T* col = F + static_cast<int64_t>(min(cl, k - 1)) * m;
+          double lv[4], tr[4][R];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int row = r0 + rl + 16 * s;
+            lv[s] = static_cast<double>(col[min(row, rend - 1)]);
+#pragma unroll
+            for (int j = 0; j < R; ++j) tr[s][j] = (row < rend && j < nr) ? ts[row * LD + j] : 0.0;
+          }
+          double acc[R];
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            acc[j] = 0.0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc[j] = fma(lv[s], tr[s][j], acc[j]);
+            acc[j] = row16_sum(acc[j]);
+          }
+          if (rl == 0 && cl < k)
+#pragma unroll
+            for (int j = 0; j < R; ++j) part[(rc * ldk + cl) * LD + j] = acc[j];
+        }
+      }
+      __syncthreads();
+      // z = t[0:k] - the partial sums (k_sbwd_comb: wave w adds row tiles
+      // [w qw, w qw + qw) in order, then the pairwise sum of the waves)
+      const int qw = (nrc + 3) / 4;
+      for (int idx = lane; idx < W * R; idx += 64) {
+        const int i = idx % W, j = idx / W;
+        double v[4];
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+          const int rb = ww * qw, re = min(nrc, rb + qw);
+          double a = 0.0;
+          if (j < nr)
+            for (int rc = rb; rc < re; ++rc) a += part[(rc * ldk + i) * LD + j];
+          v[ww] = a;
+        }
+        zs[i * LD + j] = j < nr ? ts[i * LD + j] - ((v[0] + v[2]) + (v[1] + v[3])) : 0.0;
+      }
+      __syncthreads();
+      // t[0:k] = X^T z by 64-row blocks (k_sbwd_inv; X^T is upper triangular)
+      const double* Xt = xinvt + sboff[sm.sb];
+      for (int a0 = 0; a0 < W; a0 += 64) {
+        const int i = a0 + lane;
+        if (i < W) {
+          const double* xr = Xt + i;
+          double v[R];
+          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * W]; }, zs, LD, a0, W - a0, v);
+          for (int j = 0; j < nr; ++j) ts[i * LD + j] = v[j];
+        }
+      }
+      __syncthreads();
+      for (int idx = lane; idx < k * R; idx += 64) {
+        const int i = idx % k, j = idx / k;
+        if (j < nr) x[static_cast<int64_t>(j0 + j) * n + d.start + i] = ts[i * LD + j];
+      }
+    }
+    for (int idx = lane; idx < m * R; idx += 64) {
+      const int i = idx % m, j = idx / m;
+      if (j < nr) t[static_cast<int64_t>(j0 + j) * tv + d.toff + i] = ts[i * LD + j];
+    }
+  }
+}
+
 // x[pivots] = y / D from the forward result in ys
 __global__ __launch_bounds__(256) void k_fwd_finish_ys(const FrontDesc* __restrict__ fd, const int32_t* __restrict__ fronts,
                                                        const double* __restrict__ ys, const double* __restrict__ dpiv,
@@ -1289,7 +1516,10 @@ __global__ void k_gather(const int64_t* __restrict__ map, const double* __restri
   if (i < cnt) dst[i] = src[map[i]];
 }
 
-// r = b - A x for a CSR matrix, 16 lanes per row, column per right-hand side
+// r = b - A x for a CSR matrix, 16 lanes per row, column per right-hand side;
+// RR right-hand sides per pass, so that a row of A is read once per pass
+// (every column's sum is formed in the same order whatever RR is)
+template <int RR>
 __global__ __launch_bounds__(256) void k_residual(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                   const double* __restrict__ av, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r, int64_t n,
@@ -1297,12 +1527,24 @@ __global__ __launch_bounds__(256) void k_residual(const int64_t* __restrict__ rp
   const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 16;
   const int sub = threadIdx.x % 16;
   if (row >= n) return;
-  for (int j = 0; j < nrhs; ++j) {
-    const double* xj = x + j * n;
-    double s = 0.0;
-    for (int64_t p = rp[row] + sub; p < rp[row + 1]; p += 16) s = fma(av[p], xj[ci[p]], s);
-    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-    if (sub == 0) r[j * n + row] = b[j * n + row] - s;
+  const int64_t p0 = rp[row], p1 = rp[row + 1];
+  for (int j0 = 0; j0 < nrhs; j0 += RR) {
+    const int nr = min(RR, nrhs - j0);
+    double s[RR];
+#pragma unroll
+    for (int j = 0; j < RR; ++j) s[j] = 0.0;
+    for (int64_t p = p0 + sub; p < p1; p += 16) {
+      const double a = av[p];
+      const int64_t c = ci[p];
+#pragma unroll
+      for (int j = 0; j < RR; ++j)
+        if (j < nr) s[j] = fma(a, x[(j0 + j) * n + c], s[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < RR; ++j) {
+      for (int o = 8; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 16);
+      if (sub == 0 && j < nr) r[(j0 + j) * n + row] = b[(j0 + j) * n + row] - s[j];
+    }
   }
 }
 
@@ -1542,6 +1784,27 @@ class Impl final : public GpuFactor {
         lvl_maxm_[L] = std::max<int64_t>(lvl_maxm_[L], P_.fd[f].m);
         lvl_maxk_[L] = std::max<int64_t>(lvl_maxk_[L], P_.fd[f].k);
       }
+    // levels solved by the fused small-front kernels, those whose fronts all
+    // have at most SMM rows and SMK pivots (RUSTYFEM_NO_SMALL_SOLVE=1: none,
+    // every level through the super-block kernels)
+    {
+      const char* e = std::getenv("RUSTYFEM_NO_SMALL_SOLVE");
+      const bool off = e && *e && *e != '0';
+      std::vector<std::array<int32_t, 2>> ch(static_cast<size_t>(P_.nf), std::array<int32_t, 2>{-1, -1});
+      for (int sl = 0; sl < 2; ++sl)
+        for (int32_t c : P_.child_slot_fronts[sl]) ch[P_.fd[c].parent][sl] = c;
+      std::vector<SmallFront> sf;
+      small_r_.assign(P_.levels.size(), Range{});
+      for (size_t L = 0; L < P_.levels.size(); ++L) {
+        if (off || lvl_maxm_[L] > SMM || lvl_maxk_[L] > SMK) continue;
+        const int64_t b = static_cast<int64_t>(sf.size());
+        for (int32_t f : P_.levels[L].fronts)
+          sf.push_back(SmallFront{f, P_.fd[f].k > 0 ? P_.sb_first[f] : -1, {ch[f][0], ch[f][1]}});
+        small_r_[L] = Range{b, static_cast<int64_t>(sf.size()) - b};
+        tm_.small_levels += 1;
+      }
+      d_small_ = dupload(sf, bytes_, s_);
+    }
     for (int sl = 0; sl < 2; ++sl) {
       slot_maxu_[sl].assign(P_.levels.size(), 0);
       for (size_t L = 0; L < P_.levels.size() && L < P_.child_slot_range[sl].size(); ++L) {
@@ -1603,7 +1866,7 @@ class Impl final : public GpuFactor {
     if (fgraph_) (void)hipGraphExecDestroy(fgraph_);
     void* ptrs[] = {d_fd_, d_cmap_, d_amap_, d_zero_, d_scatter_, d_ea_, d_gemm_, d_diag_, d_trsm_, d_panel_, d_srow_, d_btile_, d_l21_, d_l21c_, d_last_, d_sbt_, d_sbrow_, d_sbpart_, d_btask_, d_sbinv_, d_sboff_, d_xinv_, d_xinvt_, d_ys_, d_lf_, d_cs_[0],
                     d_cs_[1], d_arena_, d_dpiv_, d_adiag_, d_lval_, d_vbuf_, d_err_, d_dpos_, d_rp_, d_ci_, d_gmap_,
-                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_c_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_};
+                    d_av_, d_t_, d_x_, d_part_, d_b_, d_r_, d_c_, d_red_, d_nrm_, d_bc_, d_eq_, d_iso_, d_seg_, d_segsq_, d_small_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
@@ -2124,7 +2387,10 @@ class Impl final : public GpuFactor {
     tm_.refine_undone = 0;
     for (;;) {
       double* r = (steps % 2 == 0) ? w_.r : w_.c;
-      launch(k_residual, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      if (nrhs == 1)
+        launch(k_residual<1>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      else
+        launch(k_residual<8>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
       norms(r, rn);
       worst = 0.0;
       bool done = true;
@@ -2262,6 +2528,18 @@ class Impl final : public GpuFactor {
   // a small one spreads the chunks over workgroups (more parallelism at the
   // top of the tree, the factor read once per chunk).
   static constexpr int64_t kChunkAll = 512;
+  // dynamic LDS of the fused small-front kernels at level L: the front's
+  // vector (its level's longest), y or z (its most pivots) and, backward, the
+  // partial sums of the row tiles; a level that would need more than
+  // kSmallLdsMax bytes (16 right-hand sides per pass only) takes the
+  // super-block kernels
+  static constexpr size_t kSmallLdsMax = 64 * 1024;
+  template <int R>
+  size_t small_lds(int64_t L, bool bwd) const {
+    const int64_t lm = lvl_maxm_[L], lk = lvl_maxk_[L];
+    const int64_t rows = lm + lk + (bwd ? (lm + PRT - 1) / PRT * lk : 0);
+    return static_cast<size_t>(rows) * (R + 1) * sizeof(double);
+  }
   static unsigned rhs_chunks(int64_t blocks, unsigned nch) { return blocks >= kChunkAll ? 1u : nch; }
 
   // the super-block steps of one level, NW waves per workgroup
@@ -2313,6 +2591,14 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      if (small_r_[L].count > 0 && small_lds<R>(L, false) <= kSmallLdsMax) {
+        const Range& sr = small_r_[L];
+        hipLaunchKernelGGL((k_small_fwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
+                           small_lds<R>(L, false), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinv_,
+                           d_sboff_, d_dpiv_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]));
+        HIPCHK(hipGetLastError());
+        continue;
+      }
       launch_vec(k_fwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, dx, w_.t, n_, tv, nrhs);
       for (int sl = 0; sl < 2; ++sl) {
         const Range& cr = P_.child_slot_range[sl][L];
@@ -2328,6 +2614,15 @@ class Impl final : public GpuFactor {
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
+      if (small_r_[L].count > 0 && small_lds<R>(L, true) <= kSmallLdsMax) {
+        const Range& sr = small_r_[L];
+        hipLaunchKernelGGL((k_small_bwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
+                           small_lds<R>(L, true), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinvt_,
+                           d_sboff_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]),
+                           static_cast<int>(lvl_maxk_[L]));
+        HIPCHK(hipGetLastError());
+        continue;
+      }
       launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
       if (lvl_maxk_[L] <= 128)
         sb_bwd_steps<R, 4>(lp, nrhs, tv, nch);
@@ -2430,6 +2725,8 @@ class Impl final : public GpuFactor {
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): the launch_gemm variant
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
+  SmallFront* d_small_ = nullptr;
+  std::vector<Range> small_r_;  // per level: its fronts in d_small_ (count 0: not fused)
   int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (1, 4 or 16)
   SbInvTask* d_sbinv_ = nullptr;
   int64_t* d_sboff_ = nullptr;

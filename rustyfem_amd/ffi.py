@@ -205,9 +205,9 @@ _STATS_KEYS = ["factor_ms", "solve_ms", "refine_ms", "refine_steps", "rel_residu
 
 
 def _stats_dict(fn, h):
-    """rustyfem_stats / rustyfem_batch_stats as a dict (25 entries, rustyfem.h)."""
-    o = np.zeros(25)
-    fn(h, o.ctypes.data, 25)
+    """rustyfem_stats / rustyfem_batch_stats as a dict (26 entries, rustyfem.h)."""
+    o = np.zeros(26)
+    fn(h, o.ctypes.data, 26)
     d = dict(zip(_STATS_KEYS, o.tolist()))
     d["refine_history"] = [float(v) for v in o[16:20] if v >= 0]
     d["ilv_launches"] = float(o[20])
@@ -215,6 +215,7 @@ def _stats_dict(fn, h):
     d["fp64_fallback"] = bool(o[22])
     d["fallback_rel_residual"] = float(o[23])
     d["q_launches"] = float(o[24])
+    d["small_levels"] = int(o[25])
     return d
 
 

@@ -68,6 +68,18 @@ def test_batch_device_solve_and_refactor(rf, plates):
         assert np.allclose(u, 0.5 * v, rtol=1e-12, atol=0)
 
 
+def test_batch_fused_solves_match_superblock_kernels(rf, monkeypatch, plates):
+    """Every level of a batch of small plates is solved by the fused per-front
+    kernels; their solutions equal the super-block kernels' bit for bit."""
+    monkeypatch.setenv("RUSTYFEM_NO_SMALL_SOLVE", "1")
+    b_sb = rf.Batch(plates, max_refine=0)
+    monkeypatch.delenv("RUSTYFEM_NO_SMALL_SOLVE")
+    b_f = rf.Batch(plates, max_refine=0)
+    assert b_sb.stats()["small_levels"] == 0 and b_f.stats()["small_levels"] == b_f.info()["levels"]
+    for u, v in zip(b_f.solve([p.rhs for p in plates]), b_sb.solve([p.rhs for p in plates])):
+        assert np.array_equal(u, v)
+
+
 def test_batch_reports_failing_problem(rf, ora, plates):
     bad = chain_problem(30, lambda c: 2.0 - 1.3, lambda r, c: -1.0)
     probs = [plates[0], bad, plates[1]]

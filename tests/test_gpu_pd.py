@@ -174,6 +174,33 @@ def test_split_panel_matches_fused_panel(rf, monkeypatch, precision):
     assert np.array_equal(h_new.solve(p.rhs), h_old.solve(p.rhs))
 
 
+@pytest.mark.parametrize("precision", [None, "mixed"])
+@pytest.mark.parametrize("nrhs", [1, 5, 17])
+def test_fused_small_front_solves_match_superblock_kernels(rf, monkeypatch, precision, nrhs):
+    """Levels whose fronts have at most 192 rows and 128 pivots are solved by
+    one fused kernel per level and direction; it repeats the super-block
+    kernels' arithmetic in their order, so the solutions are bit-identical.
+    The 2D cantilever is fused at every level; the 9^3 block only at its third
+    level (fronts of at most 192 rows and 36 pivots), between levels of the
+    super-block kernels on both sides."""
+    for p in (meshes.cantilever_cps4(nx=40, ny=4), meshes.hex_block(9, nrhs=1, seed=3)):
+        rng = np.random.default_rng(nrhs)
+        b = rng.standard_normal((p.n, nrhs))
+        print(f"case fused solve {p.name} nrhs={nrhs} precision={precision}", flush=True)
+        monkeypatch.setenv("RUSTYFEM_NO_SMALL_SOLVE", "1")
+        h_sb, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+        monkeypatch.delenv("RUSTYFEM_NO_SMALL_SOLVE")
+        h_f, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+        assert h_sb.stats()["small_levels"] == 0
+        nsmall = h_f.stats()["small_levels"]
+        assert 0 < nsmall <= h_f.stats()["levels"]
+        if p.ndof == 2:
+            assert nsmall == h_f.stats()["levels"]
+        else:
+            assert nsmall < h_f.stats()["levels"]
+        assert np.array_equal(h_f.solve(b), h_sb.solve(b))
+
+
 def test_inverse_and_chain_superblock_solves_agree(rf, monkeypatch):
     """Super-block solves with the explicit FP64 inverse X = L_JJ^-1 against the
     in-LDS block chain on the same factor (a front with several super-blocks)."""
