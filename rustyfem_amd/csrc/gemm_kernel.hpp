@@ -339,15 +339,22 @@ __device__ __forceinline__ void gemm_tile_q(const FrontDesc* __restrict__ fd, co
       rd[j] = dpiv[d.start + c];
     }
   };
+  // A thread writes its four rows in a rotated order, row 4 rq + (rr + o) % 4
+  // at write rr, so that the 16-byte writes of 8 (and of 16) consecutive
+  // lanes land on distinct banks (rows 4 rq + rr alone repeat every 4 lanes)
+  const int rot = ((rq >> 1) + (rq >> 3)) & 3;
+  auto pick = [](const f4u& v, int i) { return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3]; };
   auto store = [&](int buf) {
     float dk[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) dk[j] = kload + 4 * kq + j < t.k1 ? static_cast<float>(rd[j]) : 0.0f;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
-      *reinterpret_cast<v4f*>(&S[buf][0][kq][4 * rq + rr][0]) =
-          v4f{ra[0][rr] * dk[0], ra[1][rr] * dk[1], ra[2][rr] * dk[2], ra[3][rr] * dk[3]};
-      *reinterpret_cast<v4f*>(&S[buf][1][kq][4 * rq + rr][0]) = v4f{rb[0][rr], rb[1][rr], rb[2][rr], rb[3][rr]};
+      const int e = (rr + rot) & 3;
+      *reinterpret_cast<v4f*>(&S[buf][0][kq][4 * rq + e][0]) =
+          v4f{pick(ra[0], e) * dk[0], pick(ra[1], e) * dk[1], pick(ra[2], e) * dk[2], pick(ra[3], e) * dk[3]};
+      *reinterpret_cast<v4f*>(&S[buf][1][kq][4 * rq + e][0]) =
+          v4f{pick(rb[0], e), pick(rb[1], e), pick(rb[2], e), pick(rb[3], e)};
     }
   };
   V4 acc[4][4];
@@ -372,34 +379,44 @@ __device__ __forceinline__ void gemm_tile_q(const FrontDesc* __restrict__ fd, co
           for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(av[bi][s], bv[bj][s], acc[bi][bj]);
     }
   };
+  // the C tile's entries of this lane, read before the last K tile's
+  // products so that their latency hides behind them
+  f4u cv[4][4];
+  auto cload = [&]() {
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      const int rowc = min(t.r0 + wm * 64 + bi * 16 + 4 * g, d.m - 1);
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        const int colc = min(t.c0 + wn * 64 + bj * 16 + l16, t.cend - 1);
+        cv[bi][bj] = *reinterpret_cast<const f4u*>(F + rowc + static_cast<int64_t>(colc) * m);
+      }
+    }
+  };
   gload(t.k0);
   store(0);
   __syncthreads();
+  // every K tile but the last, then the last with the C reads in flight
   int buf = 0;
-  for (int kb = t.k0; kb < t.k1; kb += KT) {
-    const bool more = kb + KT < t.k1;
-    if (more) gload(kb + KT);
+  for (int kb = t.k0; kb + KT < t.k1; kb += KT) {
+    gload(kb + KT);
     products(buf);
-    if (more) store(buf ^ 1);
+    store(buf ^ 1);
     __syncthreads();
     buf ^= 1;
   }
+  cload();
+  __builtin_amdgcn_sched_barrier(0);
+  products(buf);
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi) {
     const int row = t.r0 + wm * 64 + bi * 16 + 4 * g;
-    const int rowc = min(row, d.m - 1);
-    f4u cv[4];
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj) {
-      const int colc = min(t.c0 + wn * 64 + bj * 16 + l16, t.cend - 1);
-      cv[bj] = *reinterpret_cast<const f4u*>(F + rowc + static_cast<int64_t>(colc) * m);
-    }
     if (row >= d.m) continue;
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj) {
       const int col = t.c0 + wn * 64 + bj * 16 + l16;
       if (col >= t.cend) continue;
-      const f4u o = cv[bj] - f4u{acc[bi][bj][0], acc[bi][bj][1], acc[bi][bj][2], acc[bi][bj][3]};
+      const f4u o = cv[bi][bj] - f4u{acc[bi][bj][0], acc[bi][bj][1], acc[bi][bj][2], acc[bi][bj][3]};
       float* p = F + row + static_cast<int64_t>(col) * m;
       if (row + 3 < d.m) {
         *reinterpret_cast<f4u*>(p) = o;
