@@ -1019,6 +1019,182 @@ __global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict_
   }
 }
 
+// ---------------------------- multi-right-hand-side super-block products (MFMA)
+//
+// With more than four right-hand sides the two products of the super-block
+// solves that read the factor (the forward row update and the backward
+// partial sums) run on FP64 MFMA (v_mfma_f64_16x16x4f64) with 16 right-hand
+// sides as the N dimension: the factor tile is read once per 16 right-hand
+// sides and multiplied by the tensor cores instead of once per 4 with VALU
+// FMAs and lane reductions.  The sums are formed in another order than the
+// VALU kernels' (to rounding; both are FP64).  Used at the levels whose fronts
+// have more than SMK pivots (the sixteen-wave levels).
+constexpr int MR = 16;  // right-hand sides per MFMA pass
+// two doubles at an 8-byte aligned address (one 16-byte access)
+typedef double v2du __attribute__((ext_vector_type(2), aligned(8)));
+
+// forward: t[r][j] -= sum_c L[r, J0 + c] y[c][j] for the tile's 64 rows; wave w
+// owns rows r0 + 16 w + [0, 16) (the MFMA's M), the columns are its K in
+// steps of 4, staged y chunks of YC columns are B.  Four accumulators take
+// the K steps in turn (independent MFMA chains), added pairwise at the end.
+// A chunk's loads (its column of L per step, its y) are all issued before its
+// products.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sfwd_rows_m(const FrontDesc* __restrict__ fd, const SbRowTile* __restrict__ tiles,
+                                                     const T* __restrict__ arena, const double* __restrict__ ysrc,
+                                                     double* __restrict__ t, int64_t tv, int nrhs) {
+  using M = Mfma<double>;
+  constexpr int YC = 32, NST = YC / 4, YPT = YC * MR / 256;  // small chunks: few registers, many waves
+  static_assert(SRB == 64, "four waves of 16 rows");
+  __shared__ double y[YC][MR];
+  const SbRowTile tl = tiles[blockIdx.x];
+  const FrontDesc d = fd[tl.front];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int W = tl.J1 - tl.J0, g = lane >> 4, i = lane & 15;
+  const int64_t m = d.m;
+  // the lane's A row (clamped into the front)
+  const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * m + min(tl.r0 + 16 * w + i, d.m - 1);
+  const double* yf = ysrc + d.toff + tl.J0;
+  double* tf = t + d.toff;
+  for (int j0 = static_cast<int>(blockIdx.y) * MR; j0 < nrhs; j0 += MR * static_cast<int>(gridDim.y)) {
+    const int nr = min(MR, nrhs - j0);
+    M::acc_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = M::acc_t{0, 0, 0, 0};
+    for (int cb = 0; cb < W; cb += YC) {
+      const int cn = min(YC, W - cb), ns = (cn + 3) / 4;
+      // step st: lane group g takes column cb + 4 st + g
+      T a[NST];
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        const int c = min(4 * st + g, cn - 1);
+        a[st] = F[static_cast<int64_t>(cb + c) * m];
+      }
+      double yv[YPT];
+#pragma unroll
+      for (int q = 0; q < YPT; ++q) {
+        const int idx = tid + 256 * q, c = idx % YC, j = idx / YC;
+        yv[q] = (c < cn && j < nr) ? yf[static_cast<int64_t>(j0 + j) * tv + cb + c] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < YPT; ++q) {
+        const int idx = tid + 256 * q;
+        y[idx % YC][idx / YC] = yv[q];
+      }
+      __syncthreads();
+      // columns past the chunk meet y = 0 (their A is a clamped, finite entry)
+#pragma unroll
+      for (int st = 0; st < NST; ++st)
+        if (st < ns) acc[st & 3] = M::op(static_cast<double>(a[st]), y[4 * st + g][i], acc[st & 3]);
+    }
+    // register r of lane l: row 16 w + l/16 + 4 r, right-hand side j0 + l%16
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = tl.r0 + 16 * w + g + 4 * r;
+      const double v = (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
+      if (row < d.m && i < nr) tf[static_cast<int64_t>(j0 + i) * tv + row] -= v;
+    }
+  }
+}
+
+// backward: partial[tile][j][c] = sum over the tile's 64 rows of L[r, c] t[r][j];
+// the workgroup takes the tile's 64-column quarter blockIdx.z, wave w its 16
+// columns c0 + 64 z + 16 w + [0, 16) (the MFMA's M); the rows are the K: lane
+// group g reads rows 16 s + 4 g + [0, 4) of its column with one 16-byte
+// access (two in FP64) and feeds them to steps (s, e = 0..3); the
+// right-hand sides (staged t rows) are N.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sbwd_part_m(const FrontDesc* __restrict__ fd, const SbPartTile* __restrict__ tiles,
+                                                     const T* __restrict__ arena, const double* __restrict__ t,
+                                                     int64_t tv, int nrhs, double* __restrict__ partial) {
+  using M = Mfma<double>;
+  static_assert(BC == 256 && PRT == 64, "four quarters of 64 columns, 64 rows per tile");
+  constexpr int TP = MR + 4;  // row pitch of the staged t (the four lane groups on distinct banks)
+  constexpr int TPT = PRT * MR / 256;
+  __shared__ double ts[PRT][TP];
+  const SbPartTile tl = tiles[blockIdx.x];
+  const FrontDesc d = fd[tl.front];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int g = lane >> 4, i = lane & 15;
+  const int rend = min(tl.r0 + PRT, d.m);
+  const int span = tl.c1 - tl.c0;
+  const int cq = 64 * static_cast<int>(blockIdx.z);
+  if (cq >= span) return;
+  const int64_t m = d.m;
+  const double* tf = t + d.toff;
+  double* out = partial + static_cast<int64_t>(blockIdx.x) * nrhs * BC;
+  const bool active = cq + 16 * w < span;  // the wave holds a column of the tile
+  // the first chunk's t rows (issued before the factor's loads)
+  double tr[TPT];
+  auto tload = [&](int j0) {
+    const int nr = min(MR, nrhs - j0);
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+      const int idx = tid + 256 * q, r = idx % PRT, j = idx / PRT;
+      tr[q] = (tl.r0 + r < rend && j < nr) ? tf[static_cast<int64_t>(j0 + j) * tv + tl.r0 + r] : 0.0;
+    }
+  };
+  if (static_cast<int>(blockIdx.y) * MR < nrhs) tload(static_cast<int>(blockIdx.y) * MR);
+  // the lane's rows of its column: quad s starts at row r0 + 16 s + 4 g; a
+  // quad that crosses the tile's end is read entry by entry (clamped rows,
+  // whose t is 0)
+  T lv[4][4];  // [s][e]
+  {
+    const int c = min(tl.c0 + cq + 16 * w + i, tl.c1 - 1);
+    const T* col = arena + d.off + static_cast<int64_t>(c) * m;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int q0 = tl.r0 + 16 * s4 + 4 * g;
+      if (q0 + 3 < rend) {
+        if constexpr (sizeof(T) == 4) {
+          const f4u v = *reinterpret_cast<const f4u*>(col + q0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) lv[s4][e] = v[e];
+        } else {
+          const v2du v0 = *reinterpret_cast<const v2du*>(col + q0);
+          const v2du v1 = *reinterpret_cast<const v2du*>(col + q0 + 2);
+          lv[s4][0] = v0[0];
+          lv[s4][1] = v0[1];
+          lv[s4][2] = v1[0];
+          lv[s4][3] = v1[1];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lv[s4][e] = col[min(q0 + e, rend - 1)];
+      }
+    }
+  }
+  for (int j0 = static_cast<int>(blockIdx.y) * MR; j0 < nrhs; j0 += MR * static_cast<int>(gridDim.y)) {
+    const int nr = min(MR, nrhs - j0);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+      const int idx = tid + 256 * q;
+      ts[idx % PRT][idx / PRT] = tr[q];
+    }
+    __syncthreads();
+    // the next chunk's t in flight during this chunk's products
+    const int jn = j0 + MR * static_cast<int>(gridDim.y);
+    if (jn < nrhs) tload(jn);
+    if (!active) continue;
+    M::acc_t acc[2] = {M::acc_t{0, 0, 0, 0}, M::acc_t{0, 0, 0, 0}};
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        acc[e & 1] = M::op(static_cast<double>(lv[s4][e]), ts[16 * s4 + 4 * g + e][i], acc[e & 1]);
+    // register r of lane l: column cq + 16 w + l/16 + 4 r, right-hand side j0 + l%16
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int cl = cq + 16 * w + g + 4 * r;
+      if (cl < span && i < nr) out[static_cast<int64_t>(j0 + i) * BC + cl] = acc[0][r] + acc[1][r];
+    }
+  }
+}
+
 // ---------------------------------------- super-block inverses (explicit)
 //
 // X = L_JJ^-1 for every super-block, built once per factorization from the
@@ -1756,6 +1932,8 @@ class Impl final : public GpuFactor {
       legacy_panel_ = q && *q && *q != '0';
       const char* c = std::getenv("RUSTYFEM_SOLVE_CHAIN");
       chain_solve_ = c && *c && *c != '0';
+      const char* mm = std::getenv("RUSTYFEM_NO_MRHS_MFMA");
+      mrhs_mfma_ = !(mm && *mm && *mm != '0');
       const char* rb = std::getenv("RUSTYFEM_SOLVE_RHS_BLOCK");
       if (rb && *rb) rhs_block_ = std::atoi(rb);
       // RUSTYFEM_TUNE_MARGIN: a variant must be this fraction faster to be picked
@@ -2528,6 +2706,10 @@ class Impl final : public GpuFactor {
   // a small one spreads the chunks over workgroups (more parallelism at the
   // top of the tree, the factor read once per chunk).
   static constexpr int64_t kChunkAll = 512;
+  // the MFMA products for more than four right-hand sides (at the levels of
+  // sixteen-wave workgroups), unless RUSTYFEM_NO_MRHS_MFMA=1
+  bool use_mrhs(int nrhs) const { return mrhs_mfma_ && nrhs > 4; }
+  static unsigned mr_chunks(int nrhs) { return static_cast<unsigned>((nrhs + MR - 1) / MR); }
   // dynamic LDS of the fused small-front kernels at level L: the front's
   // vector (its level's longest), y or z (its most pivots) and, backward, the
   // partial sums of the row tiles; a level that would need more than
@@ -2554,7 +2736,11 @@ class Impl final : public GpuFactor {
                            nrhs);
         HIPCHK(hipGetLastError());
       }
-      if (st.rows.count > 0) {
+      if (st.rows.count > 0 && NW != 4 && use_mrhs(nrhs)) {
+        hipLaunchKernelGGL((k_sfwd_rows_m<T>), dim3(grid(st.rows.count), rhs_chunks(st.rows.count, mr_chunks(nrhs))),
+                           dim3(256), 0, s_, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
+        HIPCHK(hipGetLastError());
+      } else if (st.rows.count > 0) {
         hipLaunchKernelGGL((k_sfwd_rows<T, R, NW>), dim3(grid(st.rows.count), rhs_chunks(st.rows.count, nch)),
                            dim3(64 * NW), 0, s_, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
         HIPCHK(hipGetLastError());
@@ -2566,7 +2752,12 @@ class Impl final : public GpuFactor {
   void sb_bwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
     const int nbk = SBW / NB;
     for (const SbBwdStep& st : lp.sbwd) {
-      if (st.part.count > 0) {
+      if (st.part.count > 0 && NW != 4 && use_mrhs(nrhs)) {
+        hipLaunchKernelGGL((k_sbwd_part_m<T>),
+                           dim3(grid(st.part.count), rhs_chunks(st.part.count * 4, mr_chunks(nrhs)), BC / 64), dim3(256),
+                           0, s_, d_fd_, d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
+        HIPCHK(hipGetLastError());
+      } else if (st.part.count > 0) {
         hipLaunchKernelGGL((k_sbwd_part<T, R>), dim3(grid(st.part.count), rhs_chunks(st.part.count, nch)), dim3(256), 0,
                            s_, d_fd_, d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
         HIPCHK(hipGetLastError());
@@ -2726,6 +2917,7 @@ class Impl final : public GpuFactor {
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   SmallFront* d_small_ = nullptr;
+  bool mrhs_mfma_ = true;
   std::vector<Range> small_r_;  // per level: its fronts in d_small_ (count 0: not fused)
   int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (1, 4 or 16)
   SbInvTask* d_sbinv_ = nullptr;

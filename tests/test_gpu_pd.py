@@ -201,6 +201,28 @@ def test_fused_small_front_solves_match_superblock_kernels(rf, monkeypatch, prec
         assert np.array_equal(h_f.solve(b), h_sb.solve(b))
 
 
+@pytest.mark.parametrize("precision", [None, "mixed"])
+@pytest.mark.parametrize("nrhs", [5, 16, 33])
+def test_mfma_multi_rhs_solve_matches_valu_solve(rf, monkeypatch, precision, nrhs):
+    """More than four right-hand sides: the FP64-MFMA forward row update and
+    backward partial sums (16 right-hand sides per pass) against the VALU
+    kernels (4 per pass) on the same factor, at the levels of fronts with more
+    than 128 pivots: the same solution to FP64 rounding (the sums are formed in
+    another order), and the true residual of the FP64 factor's solve."""
+    p = meshes.hex_block(14, nrhs=1, seed=8)
+    b = np.random.default_rng(nrhs).standard_normal((p.n, nrhs))
+    print(f"case mfma multi-rhs nrhs={nrhs} precision={precision}", flush=True)
+    monkeypatch.setenv("RUSTYFEM_NO_MRHS_MFMA", "1")
+    h_v, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+    monkeypatch.delenv("RUSTYFEM_NO_MRHS_MFMA")
+    h_m, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
+    x_v = h_v.solve(b)
+    x_m = h_m.solve(b)
+    assert np.max(np.abs(x_m - x_v)) / np.max(np.abs(x_v)) < 1e-12
+    if precision is None:
+        assert np.all(relres(p, x_m, b) < 1e-12)
+
+
 def test_inverse_and_chain_superblock_solves_agree(rf, monkeypatch):
     """Super-block solves with the explicit FP64 inverse X = L_JJ^-1 against the
     in-LDS block chain on the same factor (a front with several super-blocks)."""
