@@ -1024,71 +1024,89 @@ __global__ __launch_bounds__(256) void k_sbwd_chain(const FrontDesc* __restrict_
 // With more than four right-hand sides the two products of the super-block
 // solves that read the factor (the forward row update and the backward
 // partial sums) run on FP64 MFMA (v_mfma_f64_16x16x4f64) with 16 right-hand
-// sides as the N dimension: the factor tile is read once per 16 right-hand
-// sides and multiplied by the tensor cores instead of once per 4 with VALU
-// FMAs and lane reductions.  The sums are formed in another order than the
-// VALU kernels' (to rounding; both are FP64).  Used at the levels whose fronts
-// have more than SMK pivots (the sixteen-wave levels).
+// sides as the N dimension: a tile of the factor is read once per 16
+// right-hand sides and multiplied by the matrix cores instead of once per 4
+// with VALU FMAs and lane reductions.  The sums are formed in another order
+// than the VALU kernels' (to rounding; both are FP64).  Used at every level
+// but those whose fronts fit the fused small-front kernels.  (The products
+// with the super-block inverses stay on the VALU kernels, which hold a
+// wave's share of an inverse in registers: measured faster than this
+// chunked MFMA product, 1.45 against 1.92 ms per direction at 16 RHS on the
+// 64^3 block.)
 constexpr int MR = 16;  // right-hand sides per MFMA pass
 // two doubles at an 8-byte aligned address (one 16-byte access)
 typedef double v2du __attribute__((ext_vector_type(2), aligned(8)));
 
-// forward: t[r][j] -= sum_c L[r, J0 + c] y[c][j] for the tile's 64 rows; wave w
-// owns rows r0 + 16 w + [0, 16) (the MFMA's M), the columns are its K in
-// steps of 4, staged y chunks of YC columns are B.  Four accumulators take
-// the K steps in turn (independent MFMA chains), added pairwise at the end.
-// A chunk's loads (its column of L per step, its y) are all issued before its
-// products.
+// acc += A[rows of the wave, c0:c1) B[c0:c1, j0:j0+16) for one workgroup of
+// four waves, wave w owning 16 rows (the MFMA's M): arow points at the lane's
+// row of A in column 0 (column stride lda), B[j * ldb + c] is column c of
+// right-hand side j.  The columns are the K in steps of 4 (lane group g takes
+// column 4 s + g), staged through LDS by chunks of YC columns; the next
+// chunk's loads are in flight during a chunk's products.  Step s goes to
+// accumulator s % 4 (independent MFMA chains).  c0 and c1 are uniform over
+// the workgroup.
+constexpr int YC = 32;
+template <typename TA>
+__device__ __forceinline__ void rows_times_rhs(const TA* __restrict__ arow, int64_t lda, int c0, int c1,
+                                               const double* __restrict__ B, int64_t ldb, int j0, int nr,
+                                               double (*y)[MR], Mfma<double>::acc_t (&acc)[4]) {
+  using M = Mfma<double>;
+  constexpr int NST = YC / 4, YPT = YC * MR / 256;
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, i = lane & 15;
+  TA a[NST];
+  double yv[YPT];
+  auto load = [&](int cb) {
+    const int cn = min(YC, c1 - cb);
+#pragma unroll
+    for (int st = 0; st < NST; ++st) a[st] = arow[static_cast<int64_t>(cb + min(4 * st + g, cn - 1)) * lda];
+#pragma unroll
+    for (int q = 0; q < YPT; ++q) {
+      const int idx = tid + 256 * q, c = idx % YC, j = idx / YC;
+      yv[q] = (c < cn && j < nr) ? B[static_cast<int64_t>(j0 + j) * ldb + cb + c] : 0.0;
+    }
+  };
+  if (c0 < c1) load(c0);
+  for (int cb = c0; cb < c1; cb += YC) {
+    const int ns = (min(YC, c1 - cb) + 3) / 4;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < YPT; ++q) {
+      const int idx = tid + 256 * q;
+      y[idx % YC][idx / YC] = yv[q];
+    }
+    __syncthreads();
+    double ac[NST];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) ac[st] = static_cast<double>(a[st]);
+    if (cb + YC < c1) load(cb + YC);
+    // columns past the chunk meet y = 0 (their A is a clamped, finite entry)
+#pragma unroll
+    for (int st = 0; st < NST; ++st)
+      if (st < ns) acc[st & 3] = M::op(ac[st], y[4 * st + g][i], acc[st & 3]);
+  }
+}
+
+// forward: t[r][j] -= sum_c L[r, J0 + c] y[c][j] for the tile's 64 rows
 template <typename T>
 __global__ __launch_bounds__(256) void k_sfwd_rows_m(const FrontDesc* __restrict__ fd, const SbRowTile* __restrict__ tiles,
                                                      const T* __restrict__ arena, const double* __restrict__ ysrc,
                                                      double* __restrict__ t, int64_t tv, int nrhs) {
   using M = Mfma<double>;
-  constexpr int YC = 32, NST = YC / 4, YPT = YC * MR / 256;  // small chunks: few registers, many waves
   static_assert(SRB == 64, "four waves of 16 rows");
   __shared__ double y[YC][MR];
   const SbRowTile tl = tiles[blockIdx.x];
   const FrontDesc d = fd[tl.front];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int W = tl.J1 - tl.J0, g = lane >> 4, i = lane & 15;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, i = lane & 15;
   const int64_t m = d.m;
-  // the lane's A row (clamped into the front)
-  const T* F = arena + d.off + static_cast<int64_t>(tl.J0) * m + min(tl.r0 + 16 * w + i, d.m - 1);
-  const double* yf = ysrc + d.toff + tl.J0;
+  const T* arow = arena + d.off + static_cast<int64_t>(tl.J0) * m + min(tl.r0 + 16 * w + i, d.m - 1);
   double* tf = t + d.toff;
   for (int j0 = static_cast<int>(blockIdx.y) * MR; j0 < nrhs; j0 += MR * static_cast<int>(gridDim.y)) {
     const int nr = min(MR, nrhs - j0);
     M::acc_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = M::acc_t{0, 0, 0, 0};
-    for (int cb = 0; cb < W; cb += YC) {
-      const int cn = min(YC, W - cb), ns = (cn + 3) / 4;
-      // step st: lane group g takes column cb + 4 st + g
-      T a[NST];
-#pragma unroll
-      for (int st = 0; st < NST; ++st) {
-        const int c = min(4 * st + g, cn - 1);
-        a[st] = F[static_cast<int64_t>(cb + c) * m];
-      }
-      double yv[YPT];
-#pragma unroll
-      for (int q = 0; q < YPT; ++q) {
-        const int idx = tid + 256 * q, c = idx % YC, j = idx / YC;
-        yv[q] = (c < cn && j < nr) ? yf[static_cast<int64_t>(j0 + j) * tv + cb + c] : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < YPT; ++q) {
-        const int idx = tid + 256 * q;
-        y[idx % YC][idx / YC] = yv[q];
-      }
-      __syncthreads();
-      // columns past the chunk meet y = 0 (their A is a clamped, finite entry)
-#pragma unroll
-      for (int st = 0; st < NST; ++st)
-        if (st < ns) acc[st & 3] = M::op(static_cast<double>(a[st]), y[4 * st + g][i], acc[st & 3]);
-    }
+    rows_times_rhs(arow, m, 0, tl.J1 - tl.J0, ysrc + d.toff + tl.J0, tv, j0, nr, y, acc);
     // register r of lane l: row 16 w + l/16 + 4 r, right-hand side j0 + l%16
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -1973,8 +1991,10 @@ class Impl final : public GpuFactor {
         for (int32_t c : P_.child_slot_fronts[sl]) ch[P_.fd[c].parent][sl] = c;
       std::vector<SmallFront> sf;
       small_r_.assign(P_.levels.size(), Range{});
+      small_ok_.assign(P_.levels.size(), false);
       for (size_t L = 0; L < P_.levels.size(); ++L) {
-        if (off || lvl_maxm_[L] > SMM || lvl_maxk_[L] > SMK) continue;
+        small_ok_[L] = lvl_maxm_[L] <= SMM && lvl_maxk_[L] <= SMK;
+        if (off || !small_ok_[L]) continue;
         const int64_t b = static_cast<int64_t>(sf.size());
         for (int32_t f : P_.levels[L].fronts)
           sf.push_back(SmallFront{f, P_.fd[f].k > 0 ? P_.sb_first[f] : -1, {ch[f][0], ch[f][1]}});
@@ -2706,9 +2726,10 @@ class Impl final : public GpuFactor {
   // a small one spreads the chunks over workgroups (more parallelism at the
   // top of the tree, the factor read once per chunk).
   static constexpr int64_t kChunkAll = 512;
-  // the MFMA products for more than four right-hand sides (at the levels of
-  // sixteen-wave workgroups), unless RUSTYFEM_NO_MRHS_MFMA=1
-  bool use_mrhs(int nrhs) const { return mrhs_mfma_ && nrhs > 4; }
+  // the MFMA products for more than four right-hand sides, unless
+  // RUSTYFEM_NO_MRHS_MFMA=1, at every level but those the fused small-front
+  // kernels can take (which keep the VALU order whether fused or not)
+  bool use_mrhs(int64_t L, int nrhs) const { return mrhs_mfma_ && nrhs > 4 && !small_ok_[L]; }
   static unsigned mr_chunks(int nrhs) { return static_cast<unsigned>((nrhs + MR - 1) / MR); }
   // dynamic LDS of the fused small-front kernels at level L: the front's
   // vector (its level's longest), y or z (its most pivots) and, backward, the
@@ -2726,7 +2747,7 @@ class Impl final : public GpuFactor {
 
   // the super-block steps of one level, NW waves per workgroup
   template <int R, int NW>
-  void sb_fwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
+  void sb_fwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch, bool mfma) {
     const int nbk = SBW / NB;
     for (const SbFwdStep& st : lp.sfwd) {
       if (st.chain.count > 0) {
@@ -2736,7 +2757,7 @@ class Impl final : public GpuFactor {
                            nrhs);
         HIPCHK(hipGetLastError());
       }
-      if (st.rows.count > 0 && NW != 4 && use_mrhs(nrhs)) {
+      if (st.rows.count > 0 && mfma) {
         hipLaunchKernelGGL((k_sfwd_rows_m<T>), dim3(grid(st.rows.count), rhs_chunks(st.rows.count, mr_chunks(nrhs))),
                            dim3(256), 0, s_, d_fd_, d_sbrow_ + st.rows.begin, d_arena_, w_.ys, w_.t, tv, nrhs);
         HIPCHK(hipGetLastError());
@@ -2749,10 +2770,10 @@ class Impl final : public GpuFactor {
   }
 
   template <int R, int NW>
-  void sb_bwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch) {
+  void sb_bwd_steps(const LevelPlan& lp, int nrhs, int64_t tv, unsigned nch, bool mfma) {
     const int nbk = SBW / NB;
     for (const SbBwdStep& st : lp.sbwd) {
-      if (st.part.count > 0 && NW != 4 && use_mrhs(nrhs)) {
+      if (st.part.count > 0 && mfma) {
         hipLaunchKernelGGL((k_sbwd_part_m<T>),
                            dim3(grid(st.part.count), rhs_chunks(st.part.count * 4, mr_chunks(nrhs)), BC / 64), dim3(256),
                            0, s_, d_fd_, d_sbpart_ + st.part.begin, d_arena_, w_.t, tv, nrhs, w_.part);
@@ -2797,9 +2818,9 @@ class Impl final : public GpuFactor {
                    nrhs);
       }
       if (lvl_maxk_[L] <= 128)
-        sb_fwd_steps<R, 4>(lp, nrhs, tv, nch);
+        sb_fwd_steps<R, 4>(lp, nrhs, tv, nch, use_mrhs(L, nrhs));
       else
-        sb_fwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch);
+        sb_fwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch, use_mrhs(L, nrhs));
       launch_vec(k_fwd_finish_ys, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.ys, d_dpiv_, dx, n_, tv, nrhs);
     }
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
@@ -2816,9 +2837,9 @@ class Impl final : public GpuFactor {
       }
       launch_vec(k_bwd_init, lp.fronts_r.count, nrhs, lvl_maxm_[L], d_fd_, fr, d_cmap_, dx, w_.t, n_, tv, nrhs);
       if (lvl_maxk_[L] <= 128)
-        sb_bwd_steps<R, 4>(lp, nrhs, tv, nch);
+        sb_bwd_steps<R, 4>(lp, nrhs, tv, nch, use_mrhs(L, nrhs));
       else
-        sb_bwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch);
+        sb_bwd_steps<R, sb_waves<R>()>(lp, nrhs, tv, nch, use_mrhs(L, nrhs));
       launch_vec(k_bwd_finish, lp.fronts_r.count, nrhs, lvl_maxk_[L], d_fd_, fr, w_.t, dx, n_, tv, nrhs);
     }
   }
@@ -2917,6 +2938,7 @@ class Impl final : public GpuFactor {
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   SmallFront* d_small_ = nullptr;
+  std::vector<bool> small_ok_;  // per level: its fronts fit the fused small-front kernels
   bool mrhs_mfma_ = true;
   std::vector<Range> small_r_;  // per level: its fronts in d_small_ (count 0: not fused)
   int rhs_block_ = 4;  // right-hand sides per pass of the solve kernels when nrhs > 1 (1, 4 or 16)

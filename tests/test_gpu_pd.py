@@ -206,9 +206,10 @@ def test_fused_small_front_solves_match_superblock_kernels(rf, monkeypatch, prec
 def test_mfma_multi_rhs_solve_matches_valu_solve(rf, monkeypatch, precision, nrhs):
     """More than four right-hand sides: the FP64-MFMA forward row update and
     backward partial sums (16 right-hand sides per pass) against the VALU
-    kernels (4 per pass) on the same factor, at the levels of fronts with more
-    than 128 pivots: the same solution to FP64 rounding (the sums are formed in
-    another order), and the true residual of the FP64 factor's solve."""
+    kernels (4 per pass) on the same factor, at the levels whose fronts do not
+    fit the fused small-front kernels: the same solution to FP64 rounding (the
+    sums are formed in another order), and the true residual of the FP64
+    factor's solve."""
     p = meshes.hex_block(14, nrhs=1, seed=8)
     b = np.random.default_rng(nrhs).standard_normal((p.n, nrhs))
     print(f"case mfma multi-rhs nrhs={nrhs} precision={precision}", flush=True)
