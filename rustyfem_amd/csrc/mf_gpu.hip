@@ -1710,16 +1710,31 @@ __global__ void k_gather(const int64_t* __restrict__ map, const double* __restri
   if (i < cnt) dst[i] = src[map[i]];
 }
 
-// r = b - A x for a CSR matrix, 16 lanes per row, column per right-hand side;
-// RR right-hand sides per pass, so that a row of A is read once per pass
-// (every column's sum is formed in the same order whatever RR is)
-template <int RR>
+// The sum over a 16-lane DPP row in the order of the butterfly
+// v += shfl_xor(v, o) for o = 8, 4, 2, 1, with row rotations instead of
+// permutes through LDS: after the steps for 8, ..., 2o the values repeat
+// with period 2o within the row, so lane l - o (mod 16) holds what lane
+// l ^ o holds, and every lane adds the same two operands in the same order.
+__device__ __forceinline__ double row16_sum_xor(double v) {
+  v += dpp_d<0x128>(v);  // row_ror:8
+  v += dpp_d<0x124>(v);  // row_ror:4
+  v += dpp_d<0x122>(v);  // row_ror:2
+  v += dpp_d<0x121>(v);  // row_ror:1
+  return v;
+}
+
+// r = b - A x for a CSR matrix, LPR lanes per row (16, or 4 for short rows),
+// column per right-hand side; RR right-hand sides per pass, so that a row of
+// A is read once per pass (every column's sum is formed in the same order
+// whatever RR is)
+template <int RR, int LPR>
 __global__ __launch_bounds__(256) void k_residual(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                   const double* __restrict__ av, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r, int64_t n,
                                                   int nrhs) {
-  const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 16;
-  const int sub = threadIdx.x % 16;
+  static_assert(LPR == 16 || LPR == 4, "16 or 4 lanes per row");
+  const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / LPR;
+  const int sub = threadIdx.x % LPR;
   if (row >= n) return;
   const int64_t p0 = rp[row], p1 = rp[row + 1];
   for (int j0 = 0; j0 < nrhs; j0 += RR) {
@@ -1727,7 +1742,8 @@ __global__ __launch_bounds__(256) void k_residual(const int64_t* __restrict__ rp
     double s[RR];
 #pragma unroll
     for (int j = 0; j < RR; ++j) s[j] = 0.0;
-    for (int64_t p = p0 + sub; p < p1; p += 16) {
+#pragma unroll 4
+    for (int64_t p = p0 + sub; p < p1; p += LPR) {
       const double a = av[p];
       const int64_t c = ci[p];
 #pragma unroll
@@ -1736,7 +1752,12 @@ __global__ __launch_bounds__(256) void k_residual(const int64_t* __restrict__ rp
     }
 #pragma unroll
     for (int j = 0; j < RR; ++j) {
-      for (int o = 8; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 16);
+      if constexpr (LPR == 16) {
+        s[j] = row16_sum_xor(s[j]);
+      } else {
+        s[j] += dpp_d<0x4E>(s[j]);  // quad_perm [2,3,0,1]: lane ^ 2
+        s[j] += dpp_d<0xB1>(s[j]);  // quad_perm [1,0,3,2]: lane ^ 1
+      }
       if (sub == 0 && j < nr) r[(j0 + j) * n + row] = b[(j0 + j) * n + row] - s[j];
     }
   }
@@ -2585,10 +2606,17 @@ class Impl final : public GpuFactor {
     tm_.refine_undone = 0;
     for (;;) {
       double* r = (steps % 2 == 0) ? w_.r : w_.c;
-      if (nrhs == 1)
-        launch(k_residual<1>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      // four lanes per row when the rows average fewer than 32 entries (2D
+      // elements), sixteen otherwise
+      const bool short_rows = n_ > 0 && nfull_ < 32 * n_;
+      if (nrhs == 1 && short_rows)
+        launch(k_residual<1, 4>, (n_ * 4 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      else if (nrhs == 1)
+        launch(k_residual<1, 16>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+      else if (short_rows)
+        launch(k_residual<8, 4>, (n_ * 4 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
       else
-        launch(k_residual<8>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
+        launch(k_residual<8, 16>, (n_ * 16 + 255) / 256, 256, d_rp_, d_ci_, d_av_, w_.x, w_.b, r, n_, nrhs);
       norms(r, rn);
       worst = 0.0;
       bool done = true;
