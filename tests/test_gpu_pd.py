@@ -116,6 +116,38 @@ def test_refactor_same_pattern(rf):
     assert np.allclose(x3, x2, rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_hex24_sixteen_rhs_matches_oracle(rf, ora, precision):
+    """The C3D8 block of the headline workload at 24^3 (45,000 DOF; fronts up
+    to 1,875 pivots, so several super-blocks, the second stream and fused and
+    super-block solve levels all take part) with 16 right-hand sides (the
+    pressure load and 15 random nodal loads; the MFMA multi-RHS products):
+    both arms against the oracle's FP64 solution.  The mixed arm refined to
+    the default 1e-10 meets 1e-9 on the pressure load; the random loads' rough
+    solutions amplify a 1e-10 residual to about 2e-9 in the solution, so the
+    mixed arm is also refined to 1e-12, which must bring every column within
+    5e-11 of the oracle."""
+    p = meshes.hex_block(24, nrhs=16, seed=11)
+    print(f"case hex24 16 rhs precision={precision} n={p.n}", flush=True)
+    ho, _, _ = ora.Handle.factor(p)
+    xo = ho.solve(p.rhs)
+    scale = np.max(np.abs(xo), axis=0)
+    hg, eg, _ = rf.Handle.factor(p, precision=precision)
+    assert hg is not None, rf.last_error()
+    xg = hg.solve(p.rhs)
+    assert np.all(relres(p, xg, p.rhs) <= 1e-10)
+    err = np.max(np.abs(xg - xo), axis=0) / scale
+    if precision is None:
+        assert np.all(err <= 1e-11), err
+        return
+    assert err[0] <= 1e-9, err
+    ht, _, _ = rf.Handle.factor(p, precision="mixed", refine_tol=1e-12)
+    xt = ht.solve(p.rhs)
+    assert ht.stats()["rel_residual"] <= 1e-12
+    err = np.max(np.abs(xt - xo), axis=0) / scale
+    assert np.all(err <= 5e-11), err
+
+
 @pytest.mark.parametrize("nrhs", [1, 5, 17])
 def test_many_rhs_match_oracle(rf, ora, nrhs):
     """Multi-RHS solves (passes of 1, 4 and 16 columns) against the oracle."""
