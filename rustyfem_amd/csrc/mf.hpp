@@ -80,8 +80,10 @@ struct SbRowTile { int32_t front, J0, J1, r0; };
 // backward: partial sums over rows [r0, r0 + PRT) of L[r, c0:c1]^T t[r]
 struct SbPartTile { int32_t front, r0, c0, c1; };
 // a front of a level solved by the fused small-front kernels: its super-block
-// (-1 when it has no pivots) and its children by slot (-1 when absent)
-struct SmallFront { int32_t front, sb, child[2]; };
+// (-1 when it has no pivots), its children by slot (-1 when absent) and the
+// pivot-block task of its first pivots (whose inverse V is the super-block
+// inverse when it has at most NB pivots)
+struct SmallFront { int32_t front, sb, child[2], vtask; };
 constexpr int SMM = 192;  // rows of the largest front the fused solves take
 constexpr int SMK = 128;  // and its pivots (the four-wave super-block kernels' widest)
 

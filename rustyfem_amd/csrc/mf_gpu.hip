@@ -1509,8 +1509,9 @@ template <typename T, int R>
 __global__ __launch_bounds__(64) void k_small_fwd(const FrontDesc* __restrict__ fd, const SmallFront* __restrict__ sf,
                                                   const int32_t* __restrict__ cmap, const T* __restrict__ arena,
                                                   const double* __restrict__ xinv, const int64_t* __restrict__ sboff,
-                                                  const double* __restrict__ dpiv, double* __restrict__ x,
-                                                  double* __restrict__ t, int64_t n, int64_t tv, int nrhs, int ldm) {
+                                                  const double* __restrict__ vbuf, const double* __restrict__ dpiv,
+                                                  double* __restrict__ x, double* __restrict__ t, int64_t n, int64_t tv,
+                                                  int nrhs, int ldm) {
   static_assert(SMK <= SBW && SMK == sb_maxw<4>() && SRB == 64, "one super-block per front, 64-row update tiles");
   constexpr int LD = R + 1;
   extern __shared__ double sh[];
@@ -1520,7 +1521,11 @@ __global__ __launch_bounds__(64) void k_small_fwd(const FrontDesc* __restrict__ 
   const FrontDesc d = fd[sm.front];
   const int lane = threadIdx.x;
   const int m = d.m, k = d.k, W = d.k;
-  const double* X = k > 0 ? xinv + sboff[sm.sb] : nullptr;
+  // X = L^-1 of the front's pivots: for at most NB pivots the pivot block's
+  // inverse V itself (which is what the super-block inverse holds then),
+  // column stride NB
+  const double* X = k <= 0 ? nullptr : k <= NB ? vbuf + static_cast<int64_t>(sm.vtask) * NB * NB : xinv + sboff[sm.sb];
+  const int64_t ldx = k <= NB ? NB : W;
   for (int j0 = static_cast<int>(blockIdx.y) * R; j0 < nrhs; j0 += R * static_cast<int>(gridDim.y)) {
     const int nr = min(R, nrhs - j0);
     __syncthreads();
@@ -1547,7 +1552,7 @@ __global__ __launch_bounds__(64) void k_small_fwd(const FrontDesc* __restrict__ 
         if (i < W) {
           const double* xr = X + i;
           double y[R];
-          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * W]; }, ts, LD, 0, min(W, a0 + 64), y);
+          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * ldx]; }, ts, LD, 0, min(W, a0 + 64), y);
 #pragma unroll
           for (int j = 0; j < R; ++j) ys[i * LD + j] = j < nr ? y[j] : 0.0;
         }
@@ -1588,8 +1593,9 @@ template <typename T, int R>
 __global__ __launch_bounds__(64) void k_small_bwd(const FrontDesc* __restrict__ fd, const SmallFront* __restrict__ sf,
                                                   const int32_t* __restrict__ cmap, const T* __restrict__ arena,
                                                   const double* __restrict__ xinvt, const int64_t* __restrict__ sboff,
-                                                  double* __restrict__ x, double* __restrict__ t, int64_t n, int64_t tv,
-                                                  int nrhs, int ldm, int ldk) {
+                                                  const double* __restrict__ vbuf, double* __restrict__ x,
+                                                  double* __restrict__ t, int64_t n, int64_t tv, int nrhs, int ldm,
+                                                  int ldk) {
   static_assert(SMK <= SBW && SMK == sb_maxw<4>() && SMK <= BC && PRT == 64,
                 "one super-block and one column chunk per front");
   constexpr int LD = R + 1;
@@ -1666,13 +1672,17 @@ __global__ __launch_bounds__(64) void k_small_bwd(const FrontDesc* __restrict__ 
       }
       __syncthreads();
       // t[0:k] = X^T z by 64-row blocks (k_sbwd_inv; X^T is upper triangular)
-      const double* Xt = xinvt + sboff[sm.sb];
+      // row i of X^T: for at most NB pivots column i of V (stride 1), else
+      // row i of the stored transpose (stride W)
+      const bool small_k = k <= NB;
+      const double* Xt = small_k ? vbuf + static_cast<int64_t>(sm.vtask) * NB * NB : xinvt + sboff[sm.sb];
       for (int a0 = 0; a0 < W; a0 += 64) {
         const int i = a0 + lane;
         if (i < W) {
-          const double* xr = Xt + i;
+          const double* xr = small_k ? Xt + static_cast<int64_t>(i) * NB : Xt + i;
+          const int64_t st = small_k ? 1 : W;
           double v[R];
-          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * W]; }, zs, LD, a0, W - a0, v);
+          four_wave_sums<R>([&](int c) { return xr[static_cast<int64_t>(c) * st]; }, zs, LD, a0, W - a0, v);
           for (int j = 0; j < nr; ++j) ts[i * LD + j] = v[j];
         }
       }
@@ -1960,7 +1970,6 @@ class Impl final : public GpuFactor {
     d_sbrow_ = dupload(P_.sbrow, bytes_, s_);
     d_sbpart_ = dupload(P_.sbpart, bytes_, s_);
     d_btask_ = dupload(P_.btask, bytes_, s_);
-    d_sbinv_ = dupload(P_.sbinv, bytes_, s_);
     d_sboff_ = dupload(P_.sbinv_off, bytes_, s_);
     d_xinv_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.sbinv_elems, 1)), bytes_);
     d_xinvt_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.sbinv_elems, 1)), bytes_);
@@ -2011,18 +2020,31 @@ class Impl final : public GpuFactor {
       for (int sl = 0; sl < 2; ++sl)
         for (int32_t c : P_.child_slot_fronts[sl]) ch[P_.fd[c].parent][sl] = c;
       std::vector<SmallFront> sf;
+      std::vector<char> vonly(static_cast<size_t>(P_.nf), 0);  // solved from V alone (no super-block inverse)
       small_r_.assign(P_.levels.size(), Range{});
       small_ok_.assign(P_.levels.size(), false);
       for (size_t L = 0; L < P_.levels.size(); ++L) {
         small_ok_[L] = lvl_maxm_[L] <= SMM && lvl_maxk_[L] <= SMK;
         if (off || !small_ok_[L]) continue;
         const int64_t b = static_cast<int64_t>(sf.size());
-        for (int32_t f : P_.levels[L].fronts)
-          sf.push_back(SmallFront{f, P_.fd[f].k > 0 ? P_.sb_first[f] : -1, {ch[f][0], ch[f][1]}});
+        for (int32_t f : P_.levels[L].fronts) {
+          const int32_t k = P_.fd[f].k;
+          sf.push_back(SmallFront{f, k > 0 ? P_.sb_first[f] : -1, {ch[f][0], ch[f][1]},
+                                  k > 0 ? P_.btask[P_.btask_off[f]] : -1});
+          if (k > 0 && k <= NB) vonly[f] = 1;
+        }
         small_r_[L] = Range{b, static_cast<int64_t>(sf.size()) - b};
         tm_.small_levels += 1;
       }
       d_small_ = dupload(sf, bytes_, s_);
+      // the super-block inverses the solves read: none for the fronts of at
+      // most NB pivots on the fused levels (they read V)
+      std::vector<SbInvTask> inv;
+      inv.reserve(P_.sbinv.size());
+      for (const SbInvTask& tk : P_.sbinv)
+        if (!vonly[tk.front]) inv.push_back(tk);
+      nsbinv_ = static_cast<int64_t>(inv.size());
+      d_sbinv_ = dupload(inv, bytes_, s_);
     }
     for (int sl = 0; sl < 2; ++sl) {
       slot_maxu_[sl].assign(P_.levels.size(), 0);
@@ -2253,7 +2275,7 @@ class Impl final : public GpuFactor {
     }
     // explicit inverses of the super-blocks for the solves
     if (!legacy_solve_ && !chain_solve_)
-      launch(k_sbinv<T>, static_cast<int64_t>(P_.sbinv.size()), 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
+      launch(k_sbinv<T>, nsbinv_, 256, d_fd_, d_sbinv_, d_arena_, d_vbuf_, d_btask_,
              d_sboff_, d_xinv_, d_xinvt_);
   }
 
@@ -2835,7 +2857,7 @@ class Impl final : public GpuFactor {
         const Range& sr = small_r_[L];
         hipLaunchKernelGGL((k_small_fwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
                            small_lds<R>(L, false), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinv_,
-                           d_sboff_, d_dpiv_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]));
+                           d_sboff_, d_vbuf_, d_dpiv_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]));
         HIPCHK(hipGetLastError());
         continue;
       }
@@ -2858,7 +2880,7 @@ class Impl final : public GpuFactor {
         const Range& sr = small_r_[L];
         hipLaunchKernelGGL((k_small_bwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
                            small_lds<R>(L, true), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinvt_,
-                           d_sboff_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]),
+                           d_sboff_, d_vbuf_, dx, w_.t, n_, tv, nrhs, static_cast<int>(lvl_maxm_[L]),
                            static_cast<int>(lvl_maxk_[L]));
         HIPCHK(hipGetLastError());
         continue;
@@ -2966,6 +2988,7 @@ class Impl final : public GpuFactor {
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   SmallFront* d_small_ = nullptr;
+  int64_t nsbinv_ = 0;  // super-block inverse tasks launched (d_sbinv_)
   std::vector<bool> small_ok_;  // per level: its fronts fit the fused small-front kernels
   bool mrhs_mfma_ = true;
   std::vector<Range> small_r_;  // per level: its fronts in d_small_ (count 0: not fused)
