@@ -2061,6 +2061,10 @@ class Impl final : public GpuFactor {
     d_adiag_ = dalloc<double>(n_, bytes_);
     d_lval_ = dalloc<double>(nlower_, bytes_);
     d_vbuf_ = dalloc<double>(static_cast<size_t>(std::max<int64_t>(P_.diag.size(), 1)) * NB * NB, bytes_);
+    // the pivot-block inverses are written on their jb x jb blocks only: the
+    // rest stays zero from here on (once, outside any captured graph)
+    HIPCHK(hipMemsetAsync(d_vbuf_, 0, static_cast<size_t>(std::max<int64_t>(P_.diag.size(), 1)) * NB * NB * sizeof(double),
+                          s_));
 
     d_err_ = dalloc<unsigned long long>(1, bytes_);
     // diagonal positions of the Lower pattern (first entry of every column)

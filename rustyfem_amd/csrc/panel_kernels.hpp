@@ -281,8 +281,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   block_ldlt_blk(Ad, Vs, reinterpret_cast<double*>(smem + TOFF), cbuf, dd, jb, orig,
                  static_cast<int64_t>(d.start) + t.j0, err);
   __syncthreads();
+  // only the jb x jb block: the buffer is zero outside it from its
+  // allocation on (and every pivot step of the plan writes the same jb)
   double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
-  for (int idx = tid; idx < NB * NB; idx += 256) vb[idx] = Vs[idx / NB][idx % NB];
+  for (int idx = tid; idx < NB * NB; idx += 256)
+    if (idx / NB < jb && idx % NB < jb) vb[idx] = Vs[idx / NB][idx % NB];
   if (tid < jb) dpiv[d.start + t.j0 + tid] = dd[tid];
 }
 
@@ -313,7 +316,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const double* vb = vbuf + static_cast<int64_t>(t.task) * NB * NB;
   double wv[NB * NB / 256];
 #pragma unroll
-  for (int s = 0; s < NB * NB / 256; ++s) wv[s] = vb[tid + 256 * s];
+  for (int s = 0; s < NB * NB / 256; ++s) {
+    const int idx = tid + 256 * s;  // zero outside the jb x jb block
+    wv[s] = (idx / NB < jb && idx % NB < jb) ? vb[idx] : 0.0;
+  }
   const double dv = tid < jb ? dpiv[d.start + t.j0 + tid] : 1.0;
   const int i = wave * 16 + (lane & 15);
   const int row = t.r0 + i;
