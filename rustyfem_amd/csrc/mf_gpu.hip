@@ -1792,22 +1792,38 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict_
   }
 }
 
-// sums of squares of every segment [seg[s], seg[s+1]) of every column, one block per segment
+// sums of squares of every segment [seg[s], seg[s+1]) of every column, one
+// block per segment; SR columns per pass, read side by side (their loads in
+// flight together), each summed in the same order
+constexpr int SR_SQ = 8;
 __global__ __launch_bounds__(256) void k_seg_sumsq(const double* __restrict__ v, int64_t n, int nrhs,
                                                    const int64_t* __restrict__ seg, int nseg, double* __restrict__ out) {
-  __shared__ double sh[4];
+  __shared__ double sh[4][SR_SQ];
   for (int sg = blockIdx.x; sg < nseg; sg += gridDim.x)
-    for (int j = 0; j < nrhs; ++j) {
-      double s = 0.0;
+    for (int j0 = 0; j0 < nrhs; j0 += SR_SQ) {
+      const int nr = min(SR_SQ, nrhs - j0);
+      double s[SR_SQ];
+#pragma unroll
+      for (int j = 0; j < SR_SQ; ++j) s[j] = 0.0;
       for (int64_t i = seg[sg] + threadIdx.x; i < seg[sg + 1]; i += 256) {
-        const double x = v[j * n + i];
-        s += x * x;
+#pragma unroll
+        for (int j = 0; j < SR_SQ; ++j)
+          if (j < nr) {
+            const double x = v[(j0 + j) * n + i];
+            s[j] += x * x;
+          }
       }
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+#pragma unroll
+      for (int j = 0; j < SR_SQ; ++j)
+        for (int o = 32; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 64);
       __syncthreads();
-      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+      if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < SR_SQ; ++j) sh[threadIdx.x >> 6][j] = s[j];
       __syncthreads();
-      if (threadIdx.x == 0) out[static_cast<int64_t>(sg) * nrhs + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+      if (threadIdx.x < nr)
+        out[static_cast<int64_t>(sg) * nrhs + j0 + threadIdx.x] =
+            (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
     }
 }
 
