@@ -140,6 +140,107 @@ __device__ __forceinline__ void gemm_tile(const FrontDesc* __restrict__ fd, cons
   }
 }
 
+// The 64 x 64 upper-left quarter of gemm_tile's tile, with one wave: for
+// update blocks of at most 64 rows and columns (the small fronts of the lower
+// levels), the other three waves of gemm_tile would only compute entries it
+// does not store.  The wave loads and stages the same K tiles (its lane is
+// the row, all KT columns per lane) and issues the MFMA sequence of
+// gemm_tile's wave 0 (wm = wn = 0), so the results are identical.
+template <typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm64(
+    const FrontDesc* __restrict__ fd, const GemmTile* __restrict__ tiles,
+                                               T* __restrict__ arena, const double* __restrict__ dpiv) {
+  using M = Mfma<T>;
+  using V4 = typename M::acc_t;
+  using L16 = typename M::lds16_t;
+  constexpr int E = 16 / static_cast<int>(sizeof(T));
+  constexpr int KT = M::KT;
+  constexpr int KP = KT + 16 / static_cast<int>(sizeof(T));
+  const GemmTile t = tiles[blockIdx.x];
+  const FrontDesc d = fd[t.front];
+  __shared__ __attribute__((aligned(16))) T As[64][KP];
+  __shared__ __attribute__((aligned(16))) T Bs[64][KP];
+  __shared__ __attribute__((aligned(16))) T Ds[KT];
+  const int lane = threadIdx.x;
+  T* F = arena + d.off;
+  const int64_t m = d.m;
+  const int arow = min(t.r0 + lane, d.m - 1);
+  const int brow = min(t.c0 + lane, t.cend - 1);
+  V4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = V4{0, 0, 0, 0};
+  T ra[KT], rb[KT];
+  double rd = 0.0;
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int c4 = 0; c4 < KT; ++c4) {
+      const T* col = F + static_cast<int64_t>(min(kb + c4, t.k1 - 1)) * m;
+      ra[c4] = col[arow];
+      rb[c4] = col[brow];
+    }
+    if (lane < KT) rd = dpiv[d.start + min(kb + lane, t.k1 - 1)];
+  };
+  auto store = [&](int kb) {
+#pragma unroll
+    for (int k4 = 0; k4 < KT; k4 += 4) {
+      lds_put4(&As[lane][k4], ra[k4], ra[k4 + 1], ra[k4 + 2], ra[k4 + 3]);
+      lds_put4(&Bs[lane][k4], rb[k4], rb[k4 + 1], rb[k4 + 2], rb[k4 + 3]);
+    }
+    if (lane < KT) Ds[lane] = kb + lane < t.k1 ? static_cast<T>(rd) : T(0);
+  };
+  const int g = lane >> 4, l16 = lane & 15;
+  gload(t.k0);
+  for (int kb = t.k0; kb < t.k1; kb += KT) {
+    __syncthreads();
+    store(kb);
+    __syncthreads();
+    if (kb + KT < t.k1) gload(kb + KT);
+#pragma unroll
+    for (int h = 0; h < KT / 16; ++h)
+#pragma unroll
+      for (int p = 0; p < 4 / E; ++p) {
+        const int kk = 16 * h + 4 * g + E * p;
+        L16 av[4], bv[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] = *reinterpret_cast<const L16*>(&As[bi * 16 + l16][kk]);
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) bv[bj] = *reinterpret_cast<const L16*>(&Bs[bj * 16 + l16][kk]);
+        const L16 dv = *reinterpret_cast<const L16*>(&Ds[kk]);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) av[bi] *= dv;
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = M::op(bv[bj][s], av[bi][s], acc[bi][bj]);
+      }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int row = t.r0 + bi * 16 + l16;
+    const int rowc = min(row, d.m - 1);
+    T cv[4][4];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int colc = min(t.c0 + bj * 16 + M::orow(lane, r), t.cend - 1);
+        cv[bj][r] = F[rowc + static_cast<int64_t>(colc) * m];
+      }
+    if (row >= d.m) continue;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = t.c0 + bj * 16 + M::orow(lane, r);
+        if (col < t.cend) F[row + static_cast<int64_t>(col) * m] = cv[bj][r] - acc[bi][bj][r];
+      }
+  }
+}
+
 // gemm_tile with K tiles twice as wide (64 columns in FP32, 32 in FP64: half
 // the barriers per product and twice the time for the next tile's loads to
 // land) and cheaper loads spread among the products.  A column's address is

@@ -2015,6 +2015,21 @@ class Impl final : public GpuFactor {
                                                              : 0;
       autotune_ = !(a && *a == '0') && force == 0;
       gemm_pick_.assign(P_.gemm.size(), force);
+      // GEMM launches whose tiles all hold at most 64 rows and columns to
+      // update take the one-wave 64 x 64 kernel (RUSTYFEM_NO_GEMM64=1: never)
+      const char* g64 = std::getenv("RUSTYFEM_NO_GEMM64");
+      gemm64_.assign(P_.gemm.size(), 0);
+      if (!(g64 && *g64 && *g64 != '0'))
+        for (const LevelPlan& lp : P_.levels)
+          for (const Phase& ph : lp.phases) {
+            if ((ph.kind != PhaseKind::Gemm128 && ph.kind != PhaseKind::GemmRest) || ph.r.count <= 0) continue;
+            bool fits = true;
+            for (int64_t q = ph.r.begin; q < ph.r.begin + ph.r.count && fits; ++q) {
+              const GemmTile& gt = P_.gemm[q];
+              fits = P_.fd[gt.front].m - gt.r0 <= 64 && gt.cend - gt.c0 <= 64;
+            }
+            gemm64_[ph.r.begin] = fits ? 1 : 0;
+          }
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -2476,7 +2491,10 @@ class Impl final : public GpuFactor {
     }
     if (ph.kind != PhaseKind::Gemm128 && ph.kind != PhaseKind::GemmRest)
       throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
-    launch_gemm(gemm_pick_[r.begin], r, st);
+    if (gemm64_[r.begin])
+      launch_on(st, k_gemm64<T>, r.count, 64, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    else
+      launch_gemm(gemm_pick_[r.begin], r, st);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], st));
@@ -2517,7 +2535,8 @@ class Impl final : public GpuFactor {
     std::vector<const Phase*> ph;
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
-        if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0) ph.push_back(&p);
+        if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0 && !gemm64_[p.r.begin])
+          ph.push_back(&p);
     constexpr int NV = kGemmVariants;
     std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -3005,6 +3024,7 @@ class Impl final : public GpuFactor {
   bool legacy_panel_ = false;
   bool autotune_ = true, tuned_ = false;
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): the launch_gemm variant
+  std::vector<int8_t> gemm64_;     // per GEMM launch (by its first tile): 1 = the 64 x 64 one-wave kernel
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   SmallFront* d_small_ = nullptr;

@@ -256,6 +256,22 @@ def test_mfma_multi_rhs_solve_matches_valu_solve(rf, monkeypatch, precision, nrh
         assert np.all(relres(p, x_m, b) < 1e-12)
 
 
+@pytest.mark.parametrize("precision", [None, "mixed"])
+def test_gemm64_tiles_match_gemm128_tiles(rf, monkeypatch, precision):
+    """Update blocks of at most 64 rows and columns run on the one-wave 64 x 64
+    GEMM kernel, which repeats the 128 x 128 kernel's first wave: identical D
+    and solutions with and without it (RUSTYFEM_NO_GEMM64) on a 2D tree (all
+    its update blocks are small) and on a 3D block (only some are)."""
+    for p in (meshes.cantilever_cps4(), meshes.hex_block(9, nrhs=2, seed=6)):
+        print(f"case gemm64 {p.name} precision={precision}", flush=True)
+        monkeypatch.setenv("RUSTYFEM_NO_GEMM64", "1")
+        h_old, _, d_old = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+        monkeypatch.delenv("RUSTYFEM_NO_GEMM64")
+        h_new, _, d_new = rf.Handle.factor(p, want_d=True, precision=precision, max_refine=0)
+        assert np.array_equal(d_new, d_old)
+        assert np.array_equal(h_new.solve(p.rhs), h_old.solve(p.rhs))
+
+
 def test_inverse_and_chain_superblock_solves_agree(rf, monkeypatch):
     """Super-block solves with the explicit FP64 inverse X = L_JJ^-1 against the
     in-LDS block chain on the same factor (a front with several super-blocks)."""
