@@ -146,9 +146,10 @@ int64_t rustyfem_eigen(const Handle *h, const int64_t *b_indptr, const int64_t *
  * residual that repeat reached (-1 if none ran), [24] how many GEMM launches
  * the autotuning moved to the kernel with 16-byte accesses (FP32 only),
  * [25] how many tree levels the triangular solves take through the fused
- * per-front kernels (levels whose fronts all have at most 192 rows and 128
- * pivots; RUSTYFEM_NO_SMALL_SOLVE=1 in the environment at factorization:
- * none).
+ * per-front kernels (levels whose fronts all have at most 384 rows and 128
+ * pivots; with more than four right-hand sides only those of at most 192
+ * rows run them; RUSTYFEM_NO_SMALL_SOLVE=1 in the environment at
+ * factorization: none).
  * At most 26 entries are written. */
 void rustyfem_stats(const Handle *h, double *out, size_t len);
 

@@ -84,7 +84,8 @@ struct SbPartTile { int32_t front, r0, c0, c1; };
 // pivot-block task of its first pivots (whose inverse V is the super-block
 // inverse when it has at most NB pivots)
 struct SmallFront { int32_t front, sb, child[2], vtask; };
-constexpr int SMM = 192;  // rows of the largest front the fused solves take
+constexpr int SMM = 384;       // rows of the largest front the fused solves take (at most 4 RHS)
+constexpr int SMM_MRHS = 192;  // with more than 4 RHS (the MFMA products take the larger fronts)
 constexpr int SMK = 128;  // and its pivots (the four-wave super-block kernels' widest)
 
 struct Range { int64_t begin = 0, count = 0; };

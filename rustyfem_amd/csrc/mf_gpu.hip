@@ -1465,7 +1465,8 @@ __global__ __launch_bounds__(64 * NW) void k_sbwd_inv(const FrontDesc* __restric
 
 // ------------------------------------------------ fused solves of small fronts
 //
-// Levels whose fronts all have at most SMM rows and SMK pivots (the lower
+// Levels whose fronts all have at most SMM rows (SMM_MRHS with more than
+// four right-hand sides) and SMK pivots (the lower levels of 3D trees, most
 // levels of 2D trees, every level of a batch of small plates) are solved by
 // one wave per front and one launch per level and direction, instead of five
 // or six launches of four-wave workgroups.  Each step repeats the arithmetic
@@ -2058,18 +2059,23 @@ class Impl final : public GpuFactor {
         small_ok_[L] = lvl_maxm_[L] <= SMM && lvl_maxk_[L] <= SMK;
         if (off || !small_ok_[L]) continue;
         const int64_t b = static_cast<int64_t>(sf.size());
+        // a level that every solve takes through the fused kernels (any
+        // number of right-hand sides, and room in LDS at 16 per pass) needs
+        // no super-block inverses for its fronts of at most NB pivots
+        const bool always = lvl_maxm_[L] <= SMM_MRHS && small_lds<16>(L, false) <= kSmallLdsMax &&
+                            small_lds<16>(L, true) <= kSmallLdsMax;
         for (int32_t f : P_.levels[L].fronts) {
           const int32_t k = P_.fd[f].k;
           sf.push_back(SmallFront{f, k > 0 ? P_.sb_first[f] : -1, {ch[f][0], ch[f][1]},
                                   k > 0 ? P_.btask[P_.btask_off[f]] : -1});
-          if (k > 0 && k <= NB) vonly[f] = 1;
+          if (always && k > 0 && k <= NB) vonly[f] = 1;
         }
         small_r_[L] = Range{b, static_cast<int64_t>(sf.size()) - b};
         tm_.small_levels += 1;
       }
       d_small_ = dupload(sf, bytes_, s_);
       // the super-block inverses the solves read: none for the fronts of at
-      // most NB pivots on the fused levels (they read V)
+      // most NB pivots on the levels always fused (they read V)
       std::vector<SbInvTask> inv;
       inv.reserve(P_.sbinv.size());
       for (const SbInvTask& tk : P_.sbinv)
@@ -2818,7 +2824,11 @@ class Impl final : public GpuFactor {
   // the MFMA products for more than four right-hand sides, unless
   // RUSTYFEM_NO_MRHS_MFMA=1, at every level but those the fused small-front
   // kernels can take (which keep the VALU order whether fused or not)
-  bool use_mrhs(int64_t L, int nrhs) const { return mrhs_mfma_ && nrhs > 4 && !small_ok_[L]; }
+  bool use_mrhs(int64_t L, int nrhs) const { return mrhs_mfma_ && nrhs > 4 && !fused_fits(L, nrhs); }
+  // the level's fronts fit the fused kernels for this many right-hand sides:
+  // up to SMM rows with at most four, SMM_MRHS with more (beyond that the
+  // MFMA products of the super-block path are faster)
+  bool fused_fits(int64_t L, int nrhs) const { return small_ok_[L] && (nrhs <= 4 || lvl_maxm_[L] <= SMM_MRHS); }
   static unsigned mr_chunks(int nrhs) { return static_cast<unsigned>((nrhs + MR - 1) / MR); }
   // dynamic LDS of the fused small-front kernels at level L: the front's
   // vector (its level's longest), y or z (its most pivots) and, backward, the
@@ -2892,7 +2902,7 @@ class Impl final : public GpuFactor {
     for (int64_t L = 0; L < static_cast<int64_t>(P_.levels.size()); ++L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      if (small_r_[L].count > 0 && small_lds<R>(L, false) <= kSmallLdsMax) {
+      if (small_r_[L].count > 0 && fused_fits(L, nrhs) && small_lds<R>(L, false) <= kSmallLdsMax) {
         const Range& sr = small_r_[L];
         hipLaunchKernelGGL((k_small_fwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
                            small_lds<R>(L, false), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinv_,
@@ -2915,7 +2925,7 @@ class Impl final : public GpuFactor {
     for (int64_t L = static_cast<int64_t>(P_.levels.size()) - 1; L >= 0; --L) {
       const LevelPlan& lp = P_.levels[L];
       const int32_t* fr = d_lf_ + lp.fronts_r.begin;
-      if (small_r_[L].count > 0 && small_lds<R>(L, true) <= kSmallLdsMax) {
+      if (small_r_[L].count > 0 && fused_fits(L, nrhs) && small_lds<R>(L, true) <= kSmallLdsMax) {
         const Range& sr = small_r_[L];
         hipLaunchKernelGGL((k_small_bwd<T, R>), dim3(grid(sr.count), rhs_chunks(sr.count, nch)), dim3(64),
                            small_lds<R>(L, true), s_, d_fd_, d_small_ + sr.begin, d_cmap_, d_arena_, d_xinvt_,

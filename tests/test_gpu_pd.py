@@ -207,18 +207,24 @@ def test_split_panel_matches_fused_panel(rf, monkeypatch, precision):
 
 
 @pytest.mark.parametrize("precision", [None, "mixed"])
-@pytest.mark.parametrize("nrhs", [1, 5, 17])
-def test_fused_small_front_solves_match_superblock_kernels(rf, monkeypatch, precision, nrhs):
-    """Levels whose fronts have at most 192 rows and 128 pivots are solved by
-    one fused kernel per level and direction; it repeats the super-block
-    kernels' arithmetic in their order, so the solutions are bit-identical.
-    The 2D cantilever is fused at every level; the 9^3 block only at its third
+@pytest.mark.parametrize("nrhs,block", [(1, 4), (5, 4), (17, 4), (17, 16)])
+def test_fused_small_front_solves_match_superblock_kernels(rf, monkeypatch, precision, nrhs, block):
+    """Levels whose fronts have at most 384 rows (192 with more than four
+    right-hand sides) and 128 pivots are solved by one fused kernel per level
+    and direction; it repeats the super-block kernels' arithmetic in their
+    order, so the solutions are bit-identical.  The 2D cantilever is fused at
+    every level; the 9^3 block at its seven lower levels with at most four
+    right-hand sides, and with more only at its third
     level (fronts of at most 192 rows and 36 pivots), between levels of the
-    super-block kernels on both sides."""
+    super-block kernels (MFMA products) on both sides.  With 16 right-hand
+    sides per pass (RUSTYFEM_SOLVE_RHS_BLOCK=16) a level whose fused kernels
+    would need more LDS than they may take runs the super-block kernels, which
+    need the super-block inverses of its fronts."""
+    monkeypatch.setenv("RUSTYFEM_SOLVE_RHS_BLOCK", str(block))
     for p in (meshes.cantilever_cps4(nx=40, ny=4), meshes.hex_block(9, nrhs=1, seed=3)):
         rng = np.random.default_rng(nrhs)
         b = rng.standard_normal((p.n, nrhs))
-        print(f"case fused solve {p.name} nrhs={nrhs} precision={precision}", flush=True)
+        print(f"case fused solve {p.name} nrhs={nrhs} block={block} precision={precision}", flush=True)
         monkeypatch.setenv("RUSTYFEM_NO_SMALL_SOLVE", "1")
         h_sb, _, _ = rf.Handle.factor(p, precision=precision, max_refine=0)
         monkeypatch.delenv("RUSTYFEM_NO_SMALL_SOLVE")
