@@ -2016,21 +2016,32 @@ class Impl final : public GpuFactor {
                                                              : 0;
       autotune_ = !(a && *a == '0') && force == 0;
       gemm_pick_.assign(P_.gemm.size(), force);
-      // GEMM launches whose tiles all hold at most 64 rows and columns to
-      // update take the one-wave 64 x 64 kernel (RUSTYFEM_NO_GEMM64=1: never)
+      // In a GEMM launch where most tiles update at most 64 rows and columns
+      // (small fronts), those tiles go first in its range and take the
+      // one-wave 64 x 64 kernel, the others the 128 x 128 kernels; in the
+      // other launches a few such tiles (the corners of large fronts, with
+      // long K) stay with the rest, beside which they run
+      // (RUSTYFEM_NO_GEMM64=1: every tile on the 128 x 128 kernels).  A
+      // tile's result does not depend on its place in the launch.
       const char* g64 = std::getenv("RUSTYFEM_NO_GEMM64");
       gemm64_.assign(P_.gemm.size(), 0);
-      if (!(g64 && *g64 && *g64 != '0'))
+      if (!(g64 && *g64 && *g64 != '0')) {
+        bool moved = false;
+        auto fits = [&](const GemmTile& gt) { return P_.fd[gt.front].m - gt.r0 <= 64 && gt.cend - gt.c0 <= 64; };
         for (const LevelPlan& lp : P_.levels)
           for (const Phase& ph : lp.phases) {
             if ((ph.kind != PhaseKind::Gemm128 && ph.kind != PhaseKind::GemmRest) || ph.r.count <= 0) continue;
-            bool fits = true;
-            for (int64_t q = ph.r.begin; q < ph.r.begin + ph.r.count && fits; ++q) {
-              const GemmTile& gt = P_.gemm[q];
-              fits = P_.fd[gt.front].m - gt.r0 <= 64 && gt.cend - gt.c0 <= 64;
-            }
-            gemm64_[ph.r.begin] = fits ? 1 : 0;
+            auto b = P_.gemm.begin() + ph.r.begin, e = b + ph.r.count;
+            if (2 * std::count_if(b, e, fits) < ph.r.count) continue;
+            auto mid = std::stable_partition(b, e, fits);
+            gemm64_[ph.r.begin] = static_cast<int32_t>(mid - b);
+            moved = moved || (mid != b && mid != e);
           }
+        if (moved) {
+          HIPCHK(hipMemcpyAsync(d_gemm_, P_.gemm.data(), P_.gemm.size() * sizeof(GemmTile), hipMemcpyHostToDevice, s_));
+          HIPCHK(hipStreamSynchronize(s_));
+        }
+      }
     }
     d_lf_ = dupload(P_.level_fronts, bytes_, s_);
     for (int sl = 0; sl < 2; ++sl) d_cs_[sl] = dupload(P_.child_slot_fronts[sl], bytes_, s_);
@@ -2497,10 +2508,9 @@ class Impl final : public GpuFactor {
     }
     if (ph.kind != PhaseKind::Gemm128 && ph.kind != PhaseKind::GemmRest)
       throw std::runtime_error("GEMM phase with 64-column tiles is not supported");
-    if (gemm64_[r.begin])
-      launch_on(st, k_gemm64<T>, r.count, 64, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
-    else
-      launch_gemm(gemm_pick_[r.begin], r, st);
+    const int64_t n64 = gemm64_[r.begin];
+    if (n64 > 0) launch_on(st, k_gemm64<T>, n64, 64, d_fd_, d_gemm_ + r.begin, d_arena_, d_dpiv_);
+    if (n64 < r.count) launch_gemm(gemm_pick_[r.begin], Range{r.begin + n64, r.count - n64}, st);
     tm_.gemm_flops += flops;
     if (ktime_) {
       HIPCHK(hipEventRecord(evs_[2 * nev_ + 1], st));
@@ -2541,14 +2551,16 @@ class Impl final : public GpuFactor {
     std::vector<const Phase*> ph;
     for (const LevelPlan& lp : P_.levels)
       for (const Phase& p : lp.phases)
-        if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > 0 && !gemm64_[p.r.begin])
+        if ((p.kind == PhaseKind::Gemm128 || p.kind == PhaseKind::GemmRest) && p.r.count > gemm64_[p.r.begin])
           ph.push_back(&p);
     constexpr int NV = kGemmVariants;
     std::vector<hipEvent_t> ev(ph.size() * 4 * NV);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(e0_, s_));
     for (size_t i = 0; i < ph.size(); ++i) {
-      const Range& r = ph[i]->r;
+      // the launch's tiles for the 128 x 128 kernels
+      const int64_t n64 = gemm64_[ph[i]->r.begin];
+      const Range r{ph[i]->r.begin + n64, ph[i]->r.count - n64};
       for (int v = 0; v < NV; ++v) {
         launch_gemm(v, r, s_);
         for (int k = 0; k < 4; ++k) {
@@ -3034,7 +3046,7 @@ class Impl final : public GpuFactor {
   bool legacy_panel_ = false;
   bool autotune_ = true, tuned_ = false;
   std::vector<int8_t> gemm_pick_;  // per GEMM launch (by its first tile): the launch_gemm variant
-  std::vector<int8_t> gemm64_;     // per GEMM launch (by its first tile): 1 = the 64 x 64 one-wave kernel
+  std::vector<int32_t> gemm64_;    // per GEMM launch (by its first tile): its leading tiles for the 64 x 64 kernel
   bool chain_solve_ = false;
   std::vector<int64_t> lvl_maxm_, lvl_maxk_, slot_maxu_[2];
   SmallFront* d_small_ = nullptr;

@@ -264,10 +264,11 @@ def test_mfma_multi_rhs_solve_matches_valu_solve(rf, monkeypatch, precision, nrh
 
 @pytest.mark.parametrize("precision", [None, "mixed"])
 def test_gemm64_tiles_match_gemm128_tiles(rf, monkeypatch, precision):
-    """Update blocks of at most 64 rows and columns run on the one-wave 64 x 64
-    GEMM kernel, which repeats the 128 x 128 kernel's first wave: identical D
-    and solutions with and without it (RUSTYFEM_NO_GEMM64) on a 2D tree (all
-    its update blocks are small) and on a 3D block (only some are)."""
+    """GEMM tiles that update at most 64 rows and columns run on the one-wave
+    64 x 64 kernel (first in their launch), which repeats the 128 x 128
+    kernel's first wave: identical D and solutions with and without it
+    (RUSTYFEM_NO_GEMM64) on a 2D tree (most of its update blocks are small)
+    and on a 3D block (only some are, mixed with larger ones in a launch)."""
     for p in (meshes.cantilever_cps4(), meshes.hex_block(9, nrhs=2, seed=6)):
         print(f"case gemm64 {p.name} precision={precision}", flush=True)
         monkeypatch.setenv("RUSTYFEM_NO_GEMM64", "1")
