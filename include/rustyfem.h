@@ -366,14 +366,16 @@ int32_t rustyfem_analyse(size_t n, size_t nn, const int64_t *loc, const double *
 
 /* Static hazard check of the GPU factorization schedule for this input
  * (additive, host only): 0 when no tile of any launch writes an arena entry
- * that another tile of the same launch reads or writes, and no launch that
+ * that another tile of the same launch reads or writes, every extend-add
+ * launch covers its children's contribution blocks exactly, and no launch that
  * runs on the second stream (the rest of an outer block's update) conflicts
  * with the pivot-block launches that run beside it; 1 with a description in
  * msg otherwise, -1 on an internal error.  plant = 1 adds a known race (an
  * in-place write of each pivot block by its diagonal tile), plant = 2 a known
  * conflict between the two streams (a write of the next outer block's first
- * column by the rest of the update), so that tests can check that the check
- * fires; plant = 0 checks the real schedule. */
+ * column by the rest of the update), plant = 3 a gap (an extend-add launch's
+ * first tall tile left out), so that tests can check that the check fires;
+ * plant = 0 checks the real schedule. */
 int32_t rustyfem_check_plan(size_t n, size_t nn, const int64_t *loc, const double *coords, const int64_t *indptr,
                             const int64_t *indices, const double *data, int32_t leaf, int32_t plant, char *msg,
                             size_t msg_len);

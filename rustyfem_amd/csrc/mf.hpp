@@ -28,7 +28,9 @@ constexpr int SR = 1024;      // rows per solve tile (4 per thread)
 constexpr int BC = 256;       // columns per backward tile (one per thread)
 constexpr int GT_M = 128;     // GEMM tile rows
 constexpr int ZC = 64;        // columns per arena-clearing tile
-constexpr int EA_T = 64;      // extend-add tile edge
+constexpr int EA_T = 64;      // extend-add tile edge (square tiles)
+constexpr int EA_TX = 256;    // rows of a tall extend-add tile (EA_T * EA_T / EA_TX columns)
+constexpr int EA_TALL = 1024; // contribution blocks with at least this many rows take tall tiles
 constexpr int SBW = 512;      // super-block width of the triangular solves
 constexpr int SRB = 64;       // rows per tile of the forward super-block update
 constexpr int PRT = 64;       // rows per tile of the backward super-block partial sums
@@ -43,7 +45,11 @@ struct FrontDesc {
 
 struct ZeroTile { int32_t front, c0; };
 struct ScatterTile { int64_t p0, p1; };
-struct EaTile { int32_t child, x0, y0, pad; };
+// rows [x0, x0 + rows) and columns [y0, y0 + EA_T * EA_T / rows) of a child's
+// contribution block; tall = 1: rows = EA_TX, else rows = EA_T
+struct EaTile { int32_t child, x0, y0, tall; };
+constexpr int ea_rows(int tall) { return tall ? EA_TX : EA_T; }
+constexpr int ea_cols(int tall) { return EA_T * EA_T / ea_rows(tall); }
 // C[r0:r0+128, c0:min(c0+TN, cend)) -= L[r0:, k0:k1] D[k0:k1] L[c0:, k0:k1]^T
 struct GemmTile { int32_t front, r0, c0, cend, k0, k1; };
 // one NB-wide pivot block of one front; its index is the slot of the block's

@@ -80,28 +80,29 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterTile* __restrict__
   for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += 256) arena[amap[p]] = T(val[p]);
 }
 
-// F_parent[map[x], map[y]] += U_child[x, y] for y <= x in one 64 x 64 tile
-template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fd, const EaTile* __restrict__ tiles,
-                                                    const int32_t* __restrict__ cmap, T* __restrict__ arena) {
-  const EaTile t = tiles[blockIdx.x];
-  const FrontDesc c = fd[t.child];
-  const FrontDesc p = fd[c.parent];
-  __shared__ int32_t mx[EA_T], my[EA_T];
-  const int32_t* cm = cmap + c.cmap_off;
+// F_parent[map[x], map[y]] += U_child[x, y] for y <= x in one tile of TX rows
+// x and 4096 / TX columns y (lane = row, so that a wave reads and writes
+// column segments of TX rows; TX = 256 for large blocks)
+template <typename T, int TX>
+__device__ __forceinline__ void extend_add_tile(const EaTile& t, const FrontDesc& c, const FrontDesc& p,
+                                                const int32_t* __restrict__ cm, T* __restrict__ arena,
+                                                int32_t* mx, int32_t* my) {
+  constexpr int TY = EA_T * EA_T / TX, NY = EA_T * EA_T / 256, YS = 256 / TX;
+  static_assert(TX <= 256 && TY <= 256 && NY * YS == TY, "one row per lane, 16 entries per thread");
   const int tid = threadIdx.x;
-  if (tid < EA_T) {
+  if (tid < TX) {
     const int x = t.x0 + tid;
     mx[tid] = x < c.u ? cm[x] : 0;
-  } else if (tid < 2 * EA_T) {
-    const int y = t.y0 + tid - EA_T;
-    my[tid - EA_T] = y < c.u ? cm[y] : 0;
+  }
+  if (tid < TY) {
+    const int y = t.y0 + tid;
+    my[tid] = y < c.u ? cm[y] : 0;
   }
   __syncthreads();
-  // all reads of the thread's 16 entries are issued before its writes (from
-  // clamped, valid addresses; only entries with y <= x < u are written)
-  constexpr int NY = EA_T * EA_T / 256;
-  const int lx = tid % EA_T, ly0 = tid / EA_T;
+  // all reads of the thread's 16 entries are issued before its writes, from
+  // clamped addresses inside both fronts (0 <= y0 <= yc <= u - 1 and
+  // xc <= u - 1); only entries with y <= x < u are written, each by one lane
+  const int lx = tid % TX, ly0 = tid / TX;
   const int x = t.x0 + lx;
   const int xc = min(x, c.u - 1);
   const T* src = arena + c.off + (c.k + xc) + static_cast<int64_t>(c.k) * c.m;
@@ -109,17 +110,31 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   T sv[NY], dv[NY];
 #pragma unroll
   for (int k = 0; k < NY; ++k) {
-    const int ly = ly0 + k * (256 / EA_T);
-    const int yc = min(t.y0 + ly, xc);
+    const int ly = ly0 + k * YS;
+    const int yc = max(min(t.y0 + ly, xc), t.y0);
     sv[k] = src[static_cast<int64_t>(yc) * c.m];
-    dv[k] = dst[static_cast<int64_t>(my[min(ly, yc - t.y0)]) * p.m];
+    dv[k] = dst[static_cast<int64_t>(my[yc - t.y0]) * p.m];
   }
   if (x >= c.u) return;
 #pragma unroll
   for (int k = 0; k < NY; ++k) {
-    const int ly = ly0 + k * (256 / EA_T);
+    const int ly = ly0 + k * YS;
     if (t.y0 + ly <= x) dst[static_cast<int64_t>(my[ly]) * p.m] = dv[k] + sv[k];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fd, const EaTile* __restrict__ tiles,
+                                                    const int32_t* __restrict__ cmap, T* __restrict__ arena) {
+  const EaTile t = tiles[blockIdx.x];
+  const FrontDesc c = fd[t.child];
+  const FrontDesc p = fd[c.parent];
+  __shared__ int32_t mx[EA_TX], my[EA_T];
+  const int32_t* cm = cmap + c.cmap_off;
+  if (t.tall)
+    extend_add_tile<T, EA_TX>(t, c, p, cm, arena, mx, my);
+  else
+    extend_add_tile<T, EA_T>(t, c, p, cm, arena, mx, my);
 }
 
 // ------------------------------------------------------------ pivot blocks

@@ -128,9 +128,12 @@ Plan build_plan(const Symbolic& sym, const std::vector<int64_t>& colptr, const s
         if (static_cast<int>(T[f].children.size()) <= s) continue;
         const int32_t c = static_cast<int32_t>(T[f].children[s]);
         P.child_slot_fronts[s].push_back(c);
+        // large blocks in tall tiles (their column segments 256 rows long),
+        // the rest in square ones; every entry y <= x < u in exactly one tile
         const int32_t uc = P.fd[c].u;
-        for (int32_t x0 = 0; x0 < uc; x0 += EA_T)
-          for (int32_t y0 = 0; y0 <= x0; y0 += EA_T) P.ea.push_back({c, x0, y0, 0});
+        const int32_t tall = uc >= EA_TALL ? 1 : 0, tx = ea_rows(tall), ty = ea_cols(tall);
+        for (int32_t x0 = 0; x0 < uc; x0 += tx)
+          for (int32_t y0 = 0; y0 < std::min(x0 + tx, uc); y0 += ty) P.ea.push_back({c, x0, y0, tall});
       }
       lp.ea[s] = take(b, static_cast<int64_t>(P.ea.size()));
       P.child_slot_range[s][L] = take(cb, static_cast<int64_t>(P.child_slot_fronts[s].size()));

@@ -188,15 +188,29 @@ std::string check_plan(const Plan& P, int plant) {
     // extend-add: each launch writes parent entries (map[x], map[y]); they must be distinct
     for (int s = 0; s < 2; ++s) {
       std::vector<std::pair<int64_t, int32_t>> w;  // (arena offset, tile)
+      std::vector<int32_t> kids;                   // children of the launch, for the coverage count
       for (int64_t t = lp.ea[s].begin; t < lp.ea[s].begin + lp.ea[s].count; ++t) {
         const EaTile& e = P.ea[t];
+        kids.push_back(e.child);
+        // planted: the launch's first tall tile is left out (a gap in its block)
+        if (plant == 3 && e.tall && (t == lp.ea[s].begin || !P.ea[t - 1].tall)) continue;
         const FrontDesc& c = P.fd[e.child];
         const FrontDesc& p = P.fd[c.parent];
         const int32_t* cm = P.cmap.data() + c.cmap_off;
-        for (int x = e.x0; x < std::min(e.x0 + EA_T, c.u); ++x)
-          for (int y = e.y0; y < std::min(e.y0 + EA_T, x + 1); ++y)
+        if (e.x0 < 0 || e.y0 < 0 || e.x0 >= c.u || e.y0 >= c.u || (e.tall != 0 && e.tall != 1) ||
+            e.x0 % ea_rows(e.tall) != 0 || e.y0 % ea_cols(e.tall) != 0)
+          return "extend-add level " + std::to_string(L) + " tile " + std::to_string(t) + " outside its block";
+        for (int x = e.x0; x < std::min(e.x0 + ea_rows(e.tall), c.u); ++x)
+          for (int y = e.y0; y < std::min(e.y0 + ea_cols(e.tall), x + 1); ++y)
             w.push_back({p.off + cm[x] + static_cast<int64_t>(cm[y]) * p.m, static_cast<int32_t>(t)});
       }
+      // every entry y <= x < u of each child's block is written
+      std::sort(kids.begin(), kids.end());
+      kids.erase(std::unique(kids.begin(), kids.end()), kids.end());
+      int64_t want = 0;
+      for (int32_t c : kids) want += static_cast<int64_t>(P.fd[c].u) * (P.fd[c].u + 1) / 2;
+      if (static_cast<int64_t>(w.size()) != want)
+        return "extend-add level " + std::to_string(L) + " slot " + std::to_string(s) + " does not cover its blocks";
       std::sort(w.begin(), w.end());
       for (size_t i = 1; i < w.size(); ++i)
         if (w[i].first == w[i - 1].first) {

@@ -109,6 +109,18 @@ def test_schedule_check_catches_planted_race(rf):
     assert msg.startswith("level") and "panel" in msg
 
 
+def test_extend_add_tiles_cover_blocks_exactly(rf):
+    """Contribution blocks of at least 1,024 rows are extend-added in tall tiles
+    (256 rows x 16 columns), smaller ones in 64 x 64 tiles: on a block whose top
+    children take tall tiles every entry y <= x < u of each block is written
+    exactly once per launch, and a planted gap (one tall tile left out) is
+    caught."""
+    p = meshes.hex_block(20)
+    assert rf.check_plan(p) == ""
+    msg = rf.check_plan(p, plant=3)
+    assert msg.startswith("extend-add") and "does not cover" in msg
+
+
 def test_lookahead_schedule_is_clean_and_checked(rf):
     """The rest of each outer block's update runs on a second stream beside the
     next outer block's pivot-block launches: on a block whose top fronts have
