@@ -73,17 +73,21 @@ __global__ __launch_bounds__(256) void k_zero(const FrontDesc* __restrict__ fd, 
   }
 }
 
+// the original entries of the level's pivot columns, added to what child slot
+// 0 wrote into the cleared fronts (each entry once per launch)
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(const ScatterTile* __restrict__ tiles, const int64_t* __restrict__ amap,
                                                  const double* __restrict__ val, T* __restrict__ arena) {
   const ScatterTile t = tiles[blockIdx.x];
-  for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += 256) arena[amap[p]] = T(val[p]);
+  for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += 256) arena[amap[p]] += T(val[p]);
 }
 
 // F_parent[map[x], map[y]] += U_child[x, y] for y <= x in one tile of TX rows
 // x and 4096 / TX columns y (lane = row, so that a wave reads and writes
-// column segments of TX rows; TX = 256 for large blocks)
-template <typename T, int TX>
+// column segments of TX rows; TX = 256 for large blocks).  ADD = false (child
+// slot 0, into the cleared parent before its original entries are added):
+// F_parent[map[x], map[y]] = U_child[x, y], without reading the parent.
+template <typename T, int TX, bool ADD>
 __device__ __forceinline__ void extend_add_tile(const EaTile& t, const FrontDesc& c, const FrontDesc& p,
                                                 const int32_t* __restrict__ cm, T* __restrict__ arena,
                                                 int32_t* mx, int32_t* my) {
@@ -113,17 +117,17 @@ __device__ __forceinline__ void extend_add_tile(const EaTile& t, const FrontDesc
     const int ly = ly0 + k * YS;
     const int yc = max(min(t.y0 + ly, xc), t.y0);
     sv[k] = src[static_cast<int64_t>(yc) * c.m];
-    dv[k] = dst[static_cast<int64_t>(my[yc - t.y0]) * p.m];
+    if constexpr (ADD) dv[k] = dst[static_cast<int64_t>(my[yc - t.y0]) * p.m];
   }
   if (x >= c.u) return;
 #pragma unroll
   for (int k = 0; k < NY; ++k) {
     const int ly = ly0 + k * YS;
-    if (t.y0 + ly <= x) dst[static_cast<int64_t>(my[ly]) * p.m] = dv[k] + sv[k];
+    if (t.y0 + ly <= x) dst[static_cast<int64_t>(my[ly]) * p.m] = ADD ? dv[k] + sv[k] : sv[k];
   }
 }
 
-template <typename T>
+template <typename T, bool ADD>
 __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fd, const EaTile* __restrict__ tiles,
                                                     const int32_t* __restrict__ cmap, T* __restrict__ arena) {
   const EaTile t = tiles[blockIdx.x];
@@ -132,9 +136,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   __shared__ int32_t mx[EA_TX], my[EA_T];
   const int32_t* cm = cmap + c.cmap_off;
   if (t.tall)
-    extend_add_tile<T, EA_TX>(t, c, p, cm, arena, mx, my);
+    extend_add_tile<T, EA_TX, ADD>(t, c, p, cm, arena, mx, my);
   else
-    extend_add_tile<T, EA_T>(t, c, p, cm, arena, mx, my);
+    extend_add_tile<T, EA_T, ADD>(t, c, p, cm, arena, mx, my);
 }
 
 // ------------------------------------------------------------ pivot blocks
@@ -2272,9 +2276,13 @@ class Impl final : public GpuFactor {
         HIPCHK(hipEventRecord(ev_rest_, s2_));
         rest_pending_ = true;
       }
+      // child slot 0 written into the cleared fronts, the original entries
+      // added, then child slot 1: every entry is orig + child0 + child1 as in
+      // the reference's order (the first sum is formed as child0 + orig, the
+      // same value in IEEE arithmetic), without reading the fronts for slot 0
+      launch(k_extend_add<T, false>, lp.ea[0].count, 256, d_fd_, d_ea_ + lp.ea[0].begin, d_cmap_, d_arena_);
       launch(k_scatter<T>, lp.scatter.count, 256, d_scatter_ + lp.scatter.begin, d_amap_, d_lval_, d_arena_);
-      for (int sl = 0; sl < 2; ++sl)
-        launch(k_extend_add<T>, lp.ea[sl].count, 256, d_fd_, d_ea_ + lp.ea[sl].begin, d_cmap_, d_arena_);
+      launch(k_extend_add<T, true>, lp.ea[1].count, 256, d_fd_, d_ea_ + lp.ea[1].begin, d_cmap_, d_arena_);
       for (size_t pi = 0; pi < lp.phases.size(); ++pi) {
         const Phase& ph = lp.phases[pi];
         // only the pivot-block phases may run beside the rest of an update
